@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the SPH-Erosion step on one MI355X.
+
+Workload (default): the single large scene, 262,144 particles in a column
+(s = 1/256, h = 2.15 s) dropped onto a 512 x 512 value-noise heightfield
+(6 octaves, amplitude 0.3), dt = 5e-4, erosion on (atomic), FP32 with
+compensated sums.  A step is one pass of the whole hot path (sort, density,
+forces + integrate + collisions + erosion, terrain heights) over the scene.
+
+Prints one JSON line: particle-steps/s over the timed region (inputs resident
+in HBM), the HBM roofline of the dominant kernel from HIP events on the
+library's stream, the density error against the FP64 oracle, the plain FP64
+GPU path for comparison, and the reference step timed on the host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "sph-erosion_amd"), os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_workload(name):
+    from sphe import scenes
+    if name == "cfg2":
+        return [scenes.cfg2()], "column 262,144 particles on 512^2 value-noise terrain (6 oct, amp 0.3), h=2.15/256, dt=5e-4, erosion atomic"
+    if name == "cfg1":
+        return [scenes.cfg1()], "dam break 4,096 particles on 64^2 terrain, h=0.0457, dt=1e-3, erosion atomic"
+    if name == "cfg3":
+        return scenes.cfg3(), "256 batched dam-break scenes, N log-uniform [2048, 65536], T in {64..256}"
+    if name == "cfg4":
+        return [scenes.cfg4()], "layer of 4,194,304 particles over ridged 2048^2 terrain, dt=5e-4"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def density_error(sc, ctx, sid):
+    """Max relative density error of the GPU FP32 pass against the FP64 oracle."""
+    import oracle as O
+    p = sc.params
+    fx = ctx.fixed_exponent(sid)
+    osc = O.make_scene(h=p.h, mass=p.mass, rho0=p.rest_density, k=p.stiffness, visc=p.visc,
+                       surf_tens=p.surf_tens, g=tuple(p.gravity), dt=p.dt, box_c=tuple(p.box_center),
+                       box_e=p.box_half, tnx=sc.terrain["nx"], tnz=sc.terrain["nz"],
+                       t_org=tuple(sc.terrain["origin"]), t_cell=float(np.float32(sc.terrain["cell"])),
+                       fx_exp=fx, erosion=1, kc=p.kc, ke=p.ke, kd=p.kd, band=p.band)
+    pos, vel = ctx.state_f64(sid)
+    ids = ctx.ids(sid)
+    rho = ctx.debug_density(sid)
+    ref = O.orc_step(osc, pos, vel, ctx.sediment_fx(sid), ids, ctx.terrain_volume_fx(sid),
+                     ctx.terrain_heights(sid), real=8)["rho"]
+    return float((np.abs(rho - ref) / ref).max())
+
+
+def cpu_baseline(sc, target_s=12.0):
+    """The reference step (oracle ref32: all pairs, AoS, FP32, OpenMP over i)
+    on this host's cores, over a strided sample of i (each still over all j)."""
+    import oracle as O
+    n = sc.n
+    P = np.zeros(n, dtype=O.PARTICLE_DTYPE)
+    P["Id"] = np.arange(n)
+    # the reference box is centred at 0: shift the unit box to [-0.5, 0.5]
+    P["Position"] = (sc.pos - 0.5).astype(np.float32)
+    P["Velocity"] = sc.vel.astype(np.float32)
+    prm = O.default_params(dt=sc.params.dt)
+    prm.h, prm.mass, prm.p0, prm.k = sc.params.h, sc.params.mass, sc.params.rest_density, sc.params.stiffness
+    prm.visc, prm.surf_tens, prm.len = sc.params.visc, sc.params.surf_tens, 0.5
+    prm.g[0], prm.g[1], prm.g[2] = sc.params.gravity[0], sc.params.gravity[1], sc.params.gravity[2]
+    stride = max(1, n // 256)
+    t = time.perf_counter()
+    m = O.ref32_step_sampled(P, prm, stride)
+    dt0 = time.perf_counter() - t
+    per_i = dt0 / max(m, 1)
+    want = int(target_s / max(per_i, 1e-9))
+    stride2 = max(1, n // max(want, 1))
+    t = time.perf_counter()
+    m2 = O.ref32_step_sampled(P, prm, stride2)
+    dt1 = time.perf_counter() - t
+    return dict(value=m2 / dt1, unit="particle-steps/s", cores=O.ref32_threads(), kind="port",
+                sample=f"reference step (all pairs, 3 passes over all {n} j) for every {stride2}-th i "
+                       f"= {m2} particles, {dt1:.1f} s; particle-steps/s = particles processed / time")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import sphe
+    log(f"bench: {sphe.lib().sphe_version().decode()} workload {a.workload}")
+    scs, desc = make_workload(a.workload)
+    N = sum(s.n for s in scs)
+
+    ctx = sphe.Context(sphe.FP32)
+    sids = [ctx.add_synthetic(s) for s in scs]
+    if not a.no_tune:
+        blk, rm = ctx.autotune(3)
+        log(f"bench: autotuned block={blk} xcd_remap={rm}")
+    blk, rm = ctx.get_tuning()
+    ctx.step(a.warmup)
+    ctx.sync()
+    mass0 = [ctx.mass(s) for s in sids]
+
+    # timed region: K steps, per-kernel HIP events on the library's stream
+    ctx.profile(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(a.steps)
+    ctx.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    kernels = {}
+    for name in ctx.profile_names():
+        ms, cnt = ctx.profile_read(name)
+        kernels[name] = dict(avg_ms=ms, launches=cnt, bytes=ctx.kernel_bytes(name) if name != "two_phase" else None)
+    ctx.profile(False)
+    status = ctx.status()
+    mass1 = [ctx.mass(s) for s in sids]
+    drift = max(abs((m1[0] + m1[1]) - (m0[0] + m0[1])) / (m0[0] + m0[1]) for m0, m1 in zip(mass0, mass1))
+    units_exact = all(m0[2] == m1[2] for m0, m1 in zip(mass0, mass1))
+
+    # same steps as a HIP graph (the library's default mode)
+    ctx.sync()
+    tg = time.perf_counter()
+    ctx.step(a.steps)
+    ctx.sync()
+    graph_elapsed = time.perf_counter() - tg
+
+    value = N * a.steps / elapsed
+    dom = max((k for k in kernels if kernels[k]["bytes"]), key=lambda k: kernels[k]["avg_ms"])
+    dk = kernels[dom]
+    achieved = dk["bytes"] / (dk["avg_ms"] * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        with open(PROFILE_TRAFFIC) as f:
+            tr = json.load(f)
+        traffic = tr.get(a.workload, {}).get(dom)
+    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=achieved / HBM_PEAK_GBS, traffic=traffic, kernel=dom,
+                    bytes_per_launch=dk["bytes"], avg_ms=dk["avg_ms"])
+
+    out = dict(metric="particle-steps/s at <=1e-5 rel density err vs FP64; HBM GB/s fraction",
+               value=value, unit="particle-steps/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
+               ms_per_step=1e3 * elapsed / a.steps, higher_is_better=True, scaling="weak",
+               vs_baseline=None, dtype="fp32", data="synthetic",
+               config=dict(workload=f"{a.workload}: {desc}", particles=N, scenes=len(scs),
+                           precision="fp32 compensated (Kahan)", block=blk, xcd_remap=rm),
+               roofline=roofline)
+    out["value_graph"] = N * a.steps / graph_elapsed
+    out["kernels_ms"] = {k: round(v["avg_ms"], 5) for k, v in kernels.items()}
+    out["mass_drift_rel"] = drift
+    out["mass_units_exact"] = units_exact
+    out["nonfinite"] = status
+
+    if len(scs) == 1 and N <= 300000:
+        log("bench: density error vs FP64 oracle")
+        out["density_rel_err_max"] = density_error(scs[0], ctx, sids[0])
+
+    if not a.no_fp64:
+        log("bench: FP64 plain GPU path")
+        c64 = sphe.Context(sphe.FP64)
+        for s in scs:
+            c64.add_synthetic(s)
+        c64.set_tuning(blk, rm)
+        c64.step(a.warmup)
+        c64.sync()
+        k64 = max(1, a.steps // 5)
+        t = time.perf_counter()
+        c64.step(k64)
+        c64.sync()
+        v64 = N * k64 / (time.perf_counter() - t)
+        out["fp64_value"] = v64
+        out["speedup_vs_fp64"] = value / v64
+        c64.close()
+
+    if not a.no_cpu and len(scs) == 1:
+        log("bench: CPU baseline (reference step on host cores)")
+        out["cpu_baseline"] = cpu_baseline(scs[0], a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

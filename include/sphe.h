@@ -1,0 +1,247 @@
+/*
+ * sphe.h -- C ABI of the MI355X SPH-Erosion simulation step.
+ *
+ * Drop-in boundary for the reference's FluidSystemSPH (Erosion/fluid_system.h)
+ * and the terrain part of Grid (Erosion/grid.h).  The reference has no FFI:
+ * its front end calls the header class directly (Erosion/main.cpp).  Each
+ * entry point below names the reference member it replaces; the C++ class in
+ * sphe_fluid_system.hpp maps the reference's method names onto them.
+ *
+ * Conventions: every function returns SPHE_OK (0) or a negative SPHE_E*
+ * code; sphe_last_error() gives the message of the last failure on a context.
+ * A context is not thread-safe: calls on one context must not overlap.  All
+ * device work of a context runs on one HIP stream owned by the context.
+ * Host arrays are plain pointers owned by the caller.
+ */
+#ifndef SPHE_H
+#define SPHE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPHE_OK 0
+#define SPHE_EARG (-1)    /* invalid argument */
+#define SPHE_EHIP (-2)    /* HIP runtime error */
+#define SPHE_ESTATE (-3)  /* call not valid in the current state */
+#define SPHE_ENOMEM (-4)  /* allocation failed */
+#define SPHE_ERANGE (-5)  /* index / id out of range */
+
+/* Arithmetic of the step. */
+#define SPHE_FP32 0        /* FP32, Kahan-compensated neighbour sums */
+#define SPHE_FP32_NAIVE 1  /* FP32, plain sums */
+#define SPHE_FP64 2        /* plain FP64 comparison path */
+
+/* Erosion pass. */
+#define SPHE_EROSION_OFF 0
+#define SPHE_EROSION_ATOMIC 1    /* 64-bit integer atomics on the heightfield */
+#define SPHE_EROSION_TWO_PHASE 2 /* per-particle deltas, then one reduction per terrain cell */
+
+/* Parameter ids for sphe_param_ptr (the reference's pointer getters). */
+#define SPHE_P_MASS 0      /* FluidSystemSPH::GetMass    fluid_system.h:261-264 */
+#define SPHE_P_VISC 1      /* FluidSystemSPH::GetVisc    fluid_system.h:266-269 */
+#define SPHE_P_SURF_TEN 2  /* FluidSystemSPH::GetSurfTen fluid_system.h:271-274 */
+#define SPHE_P_P0 3        /* FluidSystemSPH::Getp0      fluid_system.h:276-279 */
+#define SPHE_P_GRAV 4      /* FluidSystemSPH::GetGrav    fluid_system.h:281-284 (3 floats) */
+
+/* Per-particle buffer ids for sphe_read_buffer (the reference's FPOS..FPRESS
+ * ids, fluid_system.h:22-26, plus the sediment this build adds). */
+#define SPHE_BUF_POS 0      /* 3 floats */
+#define SPHE_BUF_VEL 1      /* 3 floats */
+#define SPHE_BUF_DENSITY 2  /* 1 float (of the last step) */
+#define SPHE_BUF_MASS 3     /* 1 float (uniform particle mass) */
+#define SPHE_BUF_PRESS 4    /* 1 float (of the last step) */
+#define SPHE_BUF_SED 5      /* 1 float, carried sediment volume */
+#define SPHE_BUF_ID 6       /* 1 int32 */
+
+/* Simulation parameters of one scene.  sphe_params_default() gives the
+ * reference's values (fluid_system.h:460-478). */
+typedef struct sphe_params {
+    float h;             /* smoothing radius                 (h, 0.0457) */
+    float mass;          /* particle mass                    (MASS, 0.02) */
+    float rest_density;  /* rest density                     (p0, 998.29) */
+    float stiffness;     /* pressure constant                (k, 3) */
+    float visc;          /* viscosity coefficient            (visc, 3.5) */
+    float surf_tens;     /* surface tension coefficient      (surf_tens, 0.0728) */
+    float gravity[3];    /* gravity                          (g, (0,-9.82,0)) */
+    float dt;            /* time step                        (deltaT, 0) */
+    float box_center[3]; /* collision cube centre            (0,0,0) */
+    float box_half;      /* collision cube half extent       (len, 0.2) */
+    /* erosion model (this build; the reference has none) */
+    int32_t erosion;     /* SPHE_EROSION_* */
+    float kc;            /* sediment capacity per (m/s) of speed, times particle volume m/rho0 */
+    float ke;            /* erode rate per step */
+    float kd;            /* deposit rate per step */
+    float band;          /* contact band above the terrain surface */
+} sphe_params;
+
+/* Heightfield of one scene: nx * nz cells of edge `cell`, cell (i, k)
+ * centred at (origin[0] + (i+0.5) cell, origin[2] + (k+0.5) cell), height
+ * heights[k * nx + i] (absolute y, >= origin[1]).  nx == 0: no terrain. */
+typedef struct sphe_terrain_desc {
+    int32_t nx, nz;
+    float origin[3];
+    float cell;
+    const float* heights;
+} sphe_terrain_desc;
+
+/* FluidParticle (fluid_system.h:49-64), 112 bytes, as GetParticle returns it. */
+typedef struct sphe_particle {
+    int32_t Id;
+    float Position[3];
+    float Velocity[3];
+    float Acceleration[3];
+    float Density;
+    float Pressure;
+    float PressureForce[3];
+    float ViscosityForce[3];
+    float GravityForce[3];
+    float SurfaceForce[3];
+    float SurfaceNormal[3];
+    int32_t NeighbId;
+} sphe_particle;
+
+typedef struct sphe_ctx sphe_ctx;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* FluidSystemSPH::FluidSystemSPH (fluid_system.h:69-72).  precision is one of
+ * SPHE_FP32, SPHE_FP32_NAIVE, SPHE_FP64.  Uses the current HIP device. */
+int sphe_create(int precision, sphe_ctx** out);
+int sphe_destroy(sphe_ctx* ctx);
+const char* sphe_last_error(const sphe_ctx* ctx);
+/* Library version string. */
+const char* sphe_version(void);
+
+void sphe_params_default(sphe_params* p);
+
+/* Adds a scene (one FluidSystemSPH + its terrain).  Scenes advance together
+ * in each sphe_step.  terrain may be NULL. */
+int sphe_add_scene(sphe_ctx* ctx, const sphe_params* p, const sphe_terrain_desc* terrain, int* scene);
+int sphe_scene_count(const sphe_ctx* ctx, int* count);
+
+/* ---- parameters -------------------------------------------------------- */
+
+int sphe_set_params(sphe_ctx* ctx, int scene, const sphe_params* p);
+int sphe_get_params(const sphe_ctx* ctx, int scene, sphe_params* p);
+/* Stable host storage of one parameter (SPHE_P_*), read at every step, so a
+ * UI may write through it between steps like the reference's getters. */
+float* sphe_param_ptr(sphe_ctx* ctx, int scene, int which);
+/* FluidSystemSPH::SetDeltaTime / GetDeltaTime (fluid_system.h:216-224) */
+int sphe_set_dt(sphe_ctx* ctx, int scene, float dt);
+int sphe_get_dt(const sphe_ctx* ctx, int scene, float* dt);
+/* FluidSystemSPH::SetOrigin / GetOrigin (fluid_system.h:206-214) */
+int sphe_set_origin(sphe_ctx* ctx, int scene, const float origin[3]);
+int sphe_get_origin(const sphe_ctx* ctx, int scene, float origin[3]);
+
+/* ---- particles --------------------------------------------------------- */
+
+/* FluidSystemSPH::Initialize (fluid_system.h:74-102): appends the reference
+ * lattice of ceil(cbrt(n))^3 particles at the scene origin. */
+int sphe_init_lattice(sphe_ctx* ctx, int scene, int n);
+/* FluidSystemSPH::AddParticles (fluid_system.h:232-251) */
+int sphe_add_particles(sphe_ctx* ctx, int scene, int n);
+/* FluidSystemSPH::Reset (fluid_system.h:253-259) */
+int sphe_reset(sphe_ctx* ctx, int scene);
+/* Replaces a scene's particles (any state; used for synthetic scenes and
+ * checkpoints).  vel, ids, sed may be NULL (zero velocity, ids 0..n-1, zero
+ * sediment).  ids must be unique and non-negative. */
+int sphe_set_particles(sphe_ctx* ctx, int scene, int n, const float* pos3, const float* vel3,
+                       const int32_t* ids, const int64_t* sed_fx);
+/* Same in FP64 (for the FP64 context). */
+int sphe_set_particles_f64(sphe_ctx* ctx, int scene, int n, const double* pos3, const double* vel3,
+                           const int32_t* ids, const int64_t* sed_fx);
+int sphe_num_particles(const sphe_ctx* ctx, int scene, int* n);
+/* Copies one per-particle buffer (SPHE_BUF_*) in ascending id order. */
+int sphe_read_buffer(sphe_ctx* ctx, int scene, int which, void* out, size_t bytes);
+/* Positions and velocities in FP64 (FP64 context) in ascending id order. */
+int sphe_read_state_f64(sphe_ctx* ctx, int scene, double* pos3, double* vel3);
+/* Fixed-point sediment (1 unit = 2^-sphe_fixed_exponent volume), id order. */
+int sphe_read_sediment_fx(sphe_ctx* ctx, int scene, int64_t* sed_fx);
+/* FluidSystemSPH::GetParticle (fluid_system.h:286-289): the 112-byte record
+ * of particle `id` after the last step (forces of that step).  NeighbId is the
+ * largest neighbour id, -1 when the particle had no neighbour. */
+int sphe_get_particle(sphe_ctx* ctx, int scene, int id, sphe_particle* out);
+
+/* ---- terrain ----------------------------------------------------------- */
+
+/* Grid::LoadHeightfield (grid.h:98-102): uint8 heights of a w x h image,
+ * map[w_index * h + h_index] as Grid::GetHeightfieldAt reads it
+ * (grid.h:104-107), scaled by `scale` and offset by the terrain origin y. */
+int sphe_terrain_load_u8(sphe_ctx* ctx, int scene, const uint8_t* img, int w, int h, float scale);
+/* Heights (absolute y) of the scene's cells, row = z. */
+int sphe_terrain_heights(sphe_ctx* ctx, int scene, float* out, size_t count);
+/* Fixed-point cell volumes, row = z. */
+int sphe_terrain_volume_fx(sphe_ctx* ctx, int scene, int64_t* out, size_t count);
+int sphe_terrain_dims(const sphe_ctx* ctx, int scene, int* nx, int* nz);
+/* Exponent e of the scene's fixed-point volume unit 2^-e. */
+int sphe_fixed_exponent(const sphe_ctx* ctx, int scene, int* e);
+/* Grid::UpdateGrid vertex stream (grid.h:138-176): 6 floats per cell
+ * (x, y, z, nx, ny, nz) of the eroded terrain, and the triangle indices of
+ * Grid::genIndices (grid.h:118-136).  Pass NULL to query sizes. */
+int sphe_terrain_mesh(sphe_ctx* ctx, int scene, float* verts, size_t* nverts_floats,
+                      uint32_t* indices, size_t* nindices);
+
+/* ---- stepping ---------------------------------------------------------- */
+
+/* FluidSystemSPH::Run (fluid_system.h:104-183) for every scene, nsteps
+ * times.  Returns after the work is queued; sphe_sync waits for it. */
+int sphe_step(sphe_ctx* ctx, int nsteps);
+int sphe_sync(sphe_ctx* ctx);
+/* Capture the step sequence in a HIP graph (default on). */
+int sphe_use_graph(sphe_ctx* ctx, int enable);
+
+/* Water and sediment bookkeeping of one scene, reduced on the GPU in FP64:
+ * terrain volume, carried sediment, and the exact mod-2^64 sum of all
+ * fixed-point volume units (conserved exactly by every step). */
+int sphe_mass(sphe_ctx* ctx, int scene, double* terrain, double* sediment, uint64_t* units);
+
+/* Non-finite state seen by the last step (positions or velocities); 0 if all
+ * finite.  The reference produces NaN velocity for a particle at rest
+ * outside the box (fluid_system.h:345); this reports it. */
+int sphe_status(sphe_ctx* ctx, uint32_t* nonfinite);
+
+/* ---- measurement ------------------------------------------------------- */
+
+/* Per-kernel HIP event timing of sphe_step (graph capture is bypassed while
+ * profiling is on). */
+int sphe_profile(sphe_ctx* ctx, int enable);
+/* Average ms and launch count of one kernel ("density", "force", ...) since
+ * profiling was enabled; names lists them comma-separated. */
+int sphe_profile_read(sphe_ctx* ctx, const char* kernel, double* avg_ms, int* launches);
+int sphe_profile_names(sphe_ctx* ctx, char* names, size_t cap);
+/* Algorithmic HBM bytes of one launch of a kernel for the current scenes. */
+int sphe_kernel_bytes(sphe_ctx* ctx, const char* kernel, double* bytes);
+
+/* ---- tuning ------------------------------------------------------------ */
+
+/* Launch parameters: block size of the neighbour kernels and the
+ * XCD-contiguous block mapping.  sphe_autotune times the step for each
+ * candidate and keeps the fastest. */
+int sphe_set_tuning(sphe_ctx* ctx, int block, int xcd_remap);
+int sphe_get_tuning(const sphe_ctx* ctx, int* block, int* xcd_remap);
+int sphe_autotune(sphe_ctx* ctx, int steps_per_trial);
+
+/* ---- verification hooks (used by the parity tests) ---------------------- */
+
+/* Cell key of every particle (x fastest within the scene grid), id order. */
+int sphe_debug_cell_keys(sphe_ctx* ctx, int scene, int32_t* keys);
+/* Grid dimensions and derived float constants of a scene:
+ * out = {h, h2, thr, t3h2, mc6, pf_scale, vf_scale, ng, cell, inv_cell,
+ *        org x, org y, org z, dims x, dims y, dims z, hq, kcv, up, dn}. */
+int sphe_debug_consts(const sphe_ctx* ctx, int scene, double* out, int cap);
+/* Neighbour sets of the grid search (self included) from the current state:
+ * counts[id order] and, if idx != NULL, the neighbour ids of each particle in
+ * ascending id order at offsets[id order].  Pass idx == NULL to get counts. */
+int sphe_debug_neighbours(sphe_ctx* ctx, int scene, int32_t* counts, const int64_t* offsets,
+                          int32_t* idx);
+/* Density of the current state (one density pass, no step), id order. */
+int sphe_debug_density(sphe_ctx* ctx, int scene, double* rho);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPHE_H */

@@ -1,0 +1,78 @@
+// sphe_device.h -- scene descriptors and device helpers shared by the HIP
+// kernels (sphe_kernels.hip) and the host runtime (sphe_runtime.cpp).
+//
+// Layout of one context in HBM (all scenes of a batch concatenated):
+//   particles  slot ranges [p_begin, p_begin + p_cap) per scene, p_cap a
+//              multiple of 64 (one wavefront); the first n slots of a scene
+//              hold particles, the rest is padding that sorts into the
+//              scene's pad cell and is skipped by every physics kernel.
+//   cells      dense uniform grid per scene, x fastest, global keys
+//              [cell_off, cell_off + ncell) plus one pad cell at
+//              cell_off + ncell; start[] is the exclusive prefix of counts.
+//   terrain    cells [t_off, t_off + tcap) per scene, row = z.
+#pragma once
+
+#include <stdint.h>
+
+#define SPHE_WAVE 64
+
+template <typename T>
+struct SceneConsts {
+    T h, h2, thr, t3h2, eps_r, inv_sqrt3;
+    T mc6;       // mass * poly6 constant 315 / (64 pi h^9)
+    T pf_scale;  // mass * spiky-gradient constant 45 / (pi h^6)
+    T vf_scale;  // visc * viscosity-laplacian constant 45 / (pi h^6)
+    T ng;        // -(poly6 gradient constant 945 / (32 pi h^9))
+    T rho0, k, mass, surf_tens, g[3], dt;
+    T box_c[3], box_lo[3], box_hi[3];
+    T inv_cell, org[3];
+    T t_org[3], t_inv, kcv, ke, kd, band, up, dn, dmax;
+};
+
+struct SceneDev {
+    int32_t p_begin, p_cap, n, tcap;
+    int32_t cell_off, ncell, dims[3];
+    int32_t t_off, tnx, tnz, erosion, fx_exp;
+    float hq;    // height = t_org[1] + (float)vol * hq
+    float t_y0;
+    int32_t pad_[2];
+    SceneConsts<float> f;
+    SceneConsts<double> d;
+};
+
+template <typename T>
+struct SceneSel;
+template <>
+struct SceneSel<float> {
+    static __host__ __device__ inline const SceneConsts<float>& get(const SceneDev& s) { return s.f; }
+};
+template <>
+struct SceneSel<double> {
+    static __host__ __device__ inline const SceneConsts<double>& get(const SceneDev& s) { return s.d; }
+};
+
+// Device pointers and sizes of one context, passed by value to launchers.
+struct DevState {
+    int64_t P;          // total particle slots (multiple of 64)
+    int64_t C;          // total cells including one pad cell per scene
+    int64_t TC;         // total terrain cells (multiple of 64 per scene)
+    int nscenes;
+    int precision;      // SPHE_FP32 / SPHE_FP32_NAIVE / SPHE_FP64
+    const SceneDev* scenes;
+    const int32_t* chunk_scene;   // scene of each 64-slot particle chunk
+    const int32_t* tchunk_scene;  // scene of each 64-cell terrain chunk
+    // state in unsorted slot order (between steps) and sorted order (in a step)
+    void* u_pos; void* u_vel; int64_t* u_sed; int32_t* u_id;
+    void* s_pos; void* s_vel; int64_t* s_sed; int32_t* s_id;
+    void* rho; void* aux;               // density; (p / rho^2, m / rho)
+    int32_t* key; uint32_t* arank;      // cell key and arrival rank of each unsorted slot
+    int32_t* tmp_id; int32_t* tmp_key; int32_t* tmp_src;
+    uint32_t* cnt; uint32_t* start; uint32_t* partial; int64_t npartial;
+    int64_t* vol; float* height;
+    int64_t* dq; int32_t* tcell;        // two-phase erosion records
+    uint32_t* tcnt; uint32_t* tstart; uint32_t* tpartial; int64_t ntpartial;
+    int64_t* tdq;                       // two-phase: deltas grouped by terrain cell
+    uint32_t* status;
+    double* red;                        // reduction scratch
+    int64_t nred;
+};

@@ -1,0 +1,873 @@
+// sphe_kernels.hip -- CDNA4 (gfx950) kernels of the SPH-Erosion step.
+//
+// One step (all scenes of a context at once), reference Erosion/fluid_system.h
+// FluidSystemSPH::Run (:104-183) + advance (:306-353):
+//   scan      counts -> cell starts (3 kernels; also re-zeroes the counts)
+//   scatter   each particle to its cell segment (arrival order)
+//   reorder   rank inside the cell by particle id -> canonical (cell, id)
+//             order; gathers the state into sorted SoA buffers
+//   density   poly6 density + pressure terms over the 27-cell stencil
+//             (fluid_system.h:109-124)
+//   force     pressure, viscosity, colour normal/Laplacian (:129-178),
+//             symplectic Euler + box response (:318-350, :355-407),
+//             heightfield collision and erosion exchange (build-defined),
+//             and the cell count of the new position for the next step
+//   heights   terrain heights from the fixed-point volumes
+// The neighbour predicate is the reference's FP32 one, glm::length(r) <= h,
+// evaluated exactly as d2 = (dx*dx + dy*dy) + dz*dz <= T(h) with T(h) the
+// largest float whose square root is <= h.  Build with -ffp-contract=off.
+
+#include <hip/hip_runtime.h>
+
+#include "sphe_device.h"
+#include "sphe_launch.h"
+
+namespace {
+
+template <typename T> struct V4s;
+template <> struct V4s<float> { typedef float4 t; };
+template <> struct V4s<double> { typedef double4 t; };
+template <typename T> using V4 = typename V4s<T>::t;
+template <typename T> struct V2s;
+template <> struct V2s<float> { typedef float2 t; };
+template <> struct V2s<double> { typedef double2 t; };
+template <typename T> using V2 = typename V2s<T>::t;
+
+__device__ __forceinline__ float d_floor(float x) { return __builtin_floorf(x); }
+__device__ __forceinline__ double d_floor(double x) { return __builtin_floor(x); }
+__device__ __forceinline__ float d_sqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double d_sqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float d_fabs(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double d_fabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float d_rint(float x) { return __builtin_rintf(x); }
+__device__ __forceinline__ double d_rint(double x) { return __builtin_rint(x); }
+__device__ __forceinline__ bool d_finite(float x) { return __builtin_isfinite(x); }
+__device__ __forceinline__ bool d_finite(double x) { return __builtin_isfinite(x); }
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8).
+// Remap so that each XCD works on one contiguous range of particle slots:
+// neighbouring cells then share that XCD's L2.  Bijective for any count.
+__device__ __forceinline__ int xcd_block(int bid, int nblk, int remap)
+{
+    if (!remap) return bid;
+    const int q = nblk >> 3, r = nblk & 7;
+    const int x = bid & 7, i = bid >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
+__device__ __forceinline__ int wave_scene(const int32_t* chunk_scene, int64_t slot)
+{
+    return __builtin_amdgcn_readfirstlane(chunk_scene[slot >> 6]);
+}
+
+template <typename T>
+__device__ __forceinline__ int cidx(T f, int nmax)
+{
+    if (!(f >= T(0))) return 0;  // negative and NaN
+    if (f >= T(nmax)) return nmax;
+    return (int)f;
+}
+
+template <typename T>
+__device__ __forceinline__ void cell_of(const SceneConsts<T>& K, const int32_t* dims, T x, T y, T z,
+                                        int& kx, int& ky, int& kz)
+{
+    kx = cidx<T>(d_floor((x - K.org[0]) * K.inv_cell), dims[0] - 1);
+    ky = cidx<T>(d_floor((y - K.org[1]) * K.inv_cell), dims[1] - 1);
+    kz = cidx<T>(d_floor((z - K.org[2]) * K.inv_cell), dims[2] - 1);
+}
+
+template <typename T, bool COMP> struct Acc;
+template <typename T> struct Acc<T, true> {
+    T s = T(0), c = T(0);
+    __device__ __forceinline__ void add(T v)
+    {
+        T y = v - c;
+        T t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+template <typename T> struct Acc<T, false> {
+    T s = T(0);
+    __device__ __forceinline__ void add(T v) { s = s + v; }
+};
+
+// Visits the candidates of the 27-cell stencil of (kx, ky, kz) in ascending
+// slot order (rows dz, dy; x-cells contiguous inside a row) and calls
+// f(j, dx, dy, dz, d2) for those inside the smoothing radius.
+template <typename T, typename F>
+__device__ __forceinline__ void visit_neighbours(const SceneDev& S, const SceneConsts<T>& K,
+                                                 const uint32_t* __restrict__ start,
+                                                 const V4<T>* __restrict__ pos,
+                                                 T xi, T yi, T zi, int kx, int ky, int kz, F&& f)
+{
+    const int nx = S.dims[0], ny = S.dims[1], nz = S.dims[2];
+    const int x0 = kx > 0 ? kx - 1 : 0;
+    const int x1 = kx < nx - 1 ? kx + 1 : nx - 1;
+    const uint32_t lo = (uint32_t)S.p_begin, hi = (uint32_t)(S.p_begin + S.n);
+    for (int dzc = -1; dzc <= 1; ++dzc) {
+        const int z = kz + dzc;
+        if (z < 0 || z >= nz) continue;
+        for (int dyc = -1; dyc <= 1; ++dyc) {
+            const int y = ky + dyc;
+            if (y < 0 || y >= ny) continue;
+            const int64_t rk = (int64_t)S.cell_off + ((int64_t)z * ny + y) * nx;
+            // rows of real cells lie inside [p_begin, p_begin + n); the clamp
+            // only matters if the counts were inconsistent
+            const uint32_t j0 = max(start[rk + x0], lo);
+            const uint32_t j1 = min(start[rk + x1 + 1], hi);
+            for (uint32_t j = j0; j < j1; ++j) {
+                const V4<T> pj = pos[j];
+                const T dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+                const T d2 = (dx * dx + dy * dy) + dz * dz;
+                if (d2 <= K.thr) f(j, dx, dy, dz, d2);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ sort
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_hash_count(DevState D)
+{
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= D.P) return;
+    const SceneDev& S = D.scenes[wave_scene(D.chunk_scene, slot)];
+    int64_t key;
+    if (slot < S.p_begin + S.n) {
+        const SceneConsts<T>& K = SceneSel<T>::get(S);
+        const V4<T> p = ((const V4<T>*)D.u_pos)[slot];
+        int kx, ky, kz;
+        cell_of<T>(K, S.dims, p.x, p.y, p.z, kx, ky, kz);
+        key = S.cell_off + ((int64_t)kz * S.dims[1] + ky) * S.dims[0] + kx;
+    } else {
+        key = S.cell_off + S.ncell;  // pad cell
+    }
+    D.key[slot] = (int32_t)key;
+    D.arank[slot] = atomicAdd(&D.cnt[key], 1u);
+}
+
+#define SCAN_ITEMS 16
+#define SCAN_TILE (256 * SCAN_ITEMS)
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uint32_t* lds)
+{
+    // wave-inclusive scan with shuffles, then a scan of the 4 wave totals
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+    const int nw = blockDim.x >> 6;
+    for (int w = 0; w < nw; ++w) {
+        uint32_t t = lds[w];
+        if (w < wave) base += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
+                                                     uint32_t* __restrict__ partial)
+{
+    __shared__ uint32_t lds[4];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+        if (base + i < n) s += in[base + i];
+    uint32_t tot;
+    block_exclusive_scan(s, &tot, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_scan_partials(uint32_t* __restrict__ partial, int64_t np)
+{
+    __shared__ uint32_t lds[4];
+    uint32_t carry = 0;
+    for (int64_t b = 0; b < np; b += 256) {
+        const int64_t i = b + threadIdx.x;
+        uint32_t v = i < np ? partial[i] : 0u;
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan(v, &tot, lds);
+        if (i < np) partial[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+// start[i] = exclusive prefix of counts; counts are reset to zero for the
+// next step's arrivals.
+__global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, uint32_t* __restrict__ start,
+                                                   int64_t n, const uint32_t* __restrict__ partial)
+{
+    __shared__ uint32_t lds[4];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        v[i] = base + i < n ? cnt[base + i] : 0u;
+        s += v[i];
+    }
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan(s, &tot, lds) + partial[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (base + i < n) {
+            start[base + i] = run;
+            cnt[base + i] = 0u;
+        }
+        run += v[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(DevState D)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D.P) return;
+    const int32_t key = D.key[p];
+    if (key < 0 || (int64_t)key >= D.C) {
+        atomicOr(D.status, 2u);
+        return;
+    }
+    const uint32_t d = D.start[key] + D.arank[p];
+    if ((int64_t)d >= D.P) {  // counts inconsistent with the slots: report, never write out of range
+        atomicOr(D.status, 2u);
+        return;
+    }
+    D.tmp_id[d] = D.u_id[p];
+    D.tmp_key[d] = key;
+    D.tmp_src[d] = (int32_t)p;
+}
+
+// Canonical order inside a cell: ascending id (ties, which only duplicate
+// ids produce, by previous slot).
+template <typename T>
+__global__ __launch_bounds__(256) void k_reorder(DevState D)
+{
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D.P) return;
+    const int32_t c = D.tmp_key[d];
+    if (c < 0 || (int64_t)c >= D.C) {
+        atomicOr(D.status, 2u);
+        return;
+    }
+    const uint32_t s0 = D.start[c];
+    const uint32_t s1 = (int64_t)D.start[c + 1] <= D.P ? D.start[c + 1] : (uint32_t)D.P;
+    const int32_t my = D.tmp_id[d];
+    const int32_t src = D.tmp_src[d];
+    uint32_t rank = 0;
+    for (uint32_t j = s0; j < s1; ++j) {
+        const int32_t o = D.tmp_id[j];
+        rank += (o < my) ? 1u : 0u;
+        if (o == my && D.tmp_src[j] < src) rank += 1u;
+    }
+    const uint32_t f = s0 + rank;
+    if ((int64_t)f >= D.P || src < 0 || (int64_t)src >= D.P) {
+        atomicOr(D.status, 2u);
+        return;
+    }
+    ((V4<T>*)D.s_pos)[f] = ((const V4<T>*)D.u_pos)[src];
+    ((V4<T>*)D.s_vel)[f] = ((const V4<T>*)D.u_vel)[src];
+    D.s_sed[f] = D.u_sed[src];
+    D.s_id[f] = my;
+}
+
+// --------------------------------------------------------------- density
+
+template <typename T, bool COMP>
+__global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap)
+{
+    const int b = xcd_block(blockIdx.x, nblk, remap);
+    const int64_t slot = (int64_t)b * blockDim.x + threadIdx.x;
+    if (slot >= D.P) return;
+    const SceneDev& S = D.scenes[wave_scene(D.chunk_scene, slot)];
+    if (slot >= S.p_begin + S.n) return;
+    const SceneConsts<T>& K = SceneSel<T>::get(S);
+    const V4<T>* __restrict__ pos = (const V4<T>*)D.s_pos;
+    const V4<T> pi = pos[slot];
+    int kx, ky, kz;
+    cell_of<T>(K, S.dims, pi.x, pi.y, pi.z, kx, ky, kz);
+    Acc<T, COMP> sum;
+    const T h2 = K.h2;
+    visit_neighbours<T>(S, K, D.start, pos, pi.x, pi.y, pi.z, kx, ky, kz,
+                        [&](uint32_t, T, T, T, T d2) {
+                            T q = h2 - d2;
+                            q = q > T(0) ? q : T(0);
+                            sum.add((q * q) * q);
+                        });
+    const T rho = sum.s * K.mc6;
+    const T p = K.k * (rho - K.rho0);
+    ((T*)D.rho)[slot] = rho;
+    V2<T> a;
+    a.x = p / (rho * rho);
+    a.y = K.mass / rho;
+    ((V2<T>*)D.aux)[slot] = a;
+}
+
+// ----------------------------------------------------------------- force
+
+template <typename T>
+struct ForceOut {
+    T pf[3], vf[3], nr[3], sf[3], gf[3], acc[3];
+    int32_t nbmax;
+    int32_t count;
+};
+
+// Pressure, viscosity, colour normal and colour Laplacian sums of one
+// particle (fluid_system.h:136-150, 165-172) with the constants factored out.
+template <typename T, bool COMP, bool INSPECT>
+__device__ __forceinline__ void force_sums(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
+                                           int64_t slot, const V4<T>& pi, const V4<T>& vi, T rho_i, T a_i,
+                                           ForceOut<T>& o)
+{
+    const V4<T>* __restrict__ pos = (const V4<T>*)D.s_pos;
+    const V4<T>* __restrict__ vel = (const V4<T>*)D.s_vel;
+    const V2<T>* __restrict__ aux = (const V2<T>*)D.aux;
+    int kx, ky, kz;
+    cell_of<T>(K, S.dims, pi.x, pi.y, pi.z, kx, ky, kz);
+    Acc<T, COMP> fp0, fp1, fp2, fv0, fv1, fv2, nn0, nn1, nn2, lap;
+    int32_t nbmax = -1, count = 0;
+    const T h = K.h, h2 = K.h2, t3h2 = K.t3h2, eps = K.eps_r, is3 = K.inv_sqrt3;
+    visit_neighbours<T>(S, K, D.start, pos, pi.x, pi.y, pi.z, kx, ky, kz,
+                        [&](uint32_t j, T dx, T dy, T dz, T d2) {
+                            T q = h2 - d2;
+                            q = q > T(0) ? q : T(0);
+                            const V2<T> aj = aux[j];
+                            const T bj = aj.y;
+                            lap.add(bj * (q * (t3h2 - T(7) * d2)));
+                            if (INSPECT) {
+                                ++count;
+                                if ((int64_t)j != slot) {
+                                    const int32_t idj = D.s_id[j];
+                                    nbmax = idj > nbmax ? idj : nbmax;
+                                }
+                            }
+                            if ((int64_t)j == slot) return;
+                            const T r = d_sqrt(d2);
+                            const T hr = h - r;
+                            T ux, uy, uz;
+                            if (r < eps) {
+                                ux = is3; uy = is3; uz = is3;
+                            } else {
+                                const T inv = T(1) / r;
+                                ux = dx * inv; uy = dy * inv; uz = dz * inv;
+                            }
+                            const T sp = (a_i + aj.x) * (hr * hr);
+                            fp0.add(sp * ux);
+                            fp1.add(sp * uy);
+                            fp2.add(sp * uz);
+                            const V4<T> vj = vel[j];
+                            const T wv = bj * hr;
+                            fv0.add((vj.x - vi.x) * wv);
+                            fv1.add((vj.y - vi.y) * wv);
+                            fv2.add((vj.z - vi.z) * wv);
+                            const T wn = bj * (q * q);
+                            nn0.add(dx * wn);
+                            nn1.add(dy * wn);
+                            nn2.add(dz * wn);
+                        });
+    const T sPF = K.pf_scale * rho_i;
+    o.pf[0] = fp0.s * sPF; o.pf[1] = fp1.s * sPF; o.pf[2] = fp2.s * sPF;
+    o.vf[0] = fv0.s * K.vf_scale; o.vf[1] = fv1.s * K.vf_scale; o.vf[2] = fv2.s * K.vf_scale;
+    o.nr[0] = nn0.s * K.ng; o.nr[1] = nn1.s * K.ng; o.nr[2] = nn2.s * K.ng;
+    const T lapv = lap.s * K.ng;
+    const T ss = (-K.surf_tens) * lapv;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        o.sf[a] = o.nr[a] * ss;
+        o.gf[a] = rho_i * K.g[a];
+        const T F = (o.pf[a] + o.vf[a]) + (o.gf[a] + o.sf[a]);
+        o.acc[a] = F / rho_i;
+    }
+    o.nbmax = nbmax;
+    o.count = count;
+}
+
+// Dominant-axis box response (fluid_system.h:342-347 with collisionS
+// :355-407), generalised to a cube of centre c and half extent e.
+template <typename T>
+__device__ __forceinline__ void box_response(const SceneConsts<T>& K, T* p, T* v)
+{
+    if ((K.box_lo[0] < p[0] && p[0] < K.box_hi[0]) && (K.box_lo[1] < p[1] && p[1] < K.box_hi[1]) &&
+        (K.box_lo[2] < p[2] && p[2] < K.box_hi[2]))
+        return;
+    int ax = 0;
+    T maxP = d_fabs(p[0] - K.box_c[0]);
+    const T u1 = d_fabs(p[1] - K.box_c[1]);
+    if (maxP < u1) { ax = 1; maxP = u1; }
+    const T u2 = d_fabs(p[2] - K.box_c[2]);
+    if (maxP < u2) { ax = 2; maxP = u2; }
+    T cp[3] = {p[0], p[1], p[2]};
+    T nr[3] = {T(0), T(0), T(0)};
+    if (p[ax] < K.box_lo[ax]) { cp[ax] = K.box_lo[ax]; nr[ax] = T(1); }
+    else { cp[ax] = K.box_hi[ax]; nr[ax] = T(-1); }
+    const T d0 = p[0] - cp[0], d1 = p[1] - cp[1], d2 = p[2] - cp[2];
+    const T d = d_sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+    const T vl = d_sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    const T s = T(1) + (T(0.5) * d) / (K.dt * vl);
+    const T vn = (v[0] * nr[0] + v[1] * nr[1]) + v[2] * nr[2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a] = v[a] - (nr[a] * s) * vn;
+    p[0] = cp[0]; p[1] = cp[1]; p[2] = cp[2];
+}
+
+// Bilinear height of the cell-centred terrain at (x, z) and the unit normal
+// of that patch (build-defined).
+template <typename T>
+__device__ __forceinline__ T terrain_sample(const SceneDev& S, const SceneConsts<T>& K, const float* __restrict__ H,
+                                            T x, T z, T* nrm)
+{
+    const int tnx = S.tnx, tnz = S.tnz;
+    const T u = (x - K.t_org[0]) * K.t_inv - T(0.5);
+    const T w = (z - K.t_org[2]) * K.t_inv - T(0.5);
+    const int i0 = cidx<T>(d_floor(u), tnx > 1 ? tnx - 2 : 0);
+    const int k0 = cidx<T>(d_floor(w), tnz > 1 ? tnz - 2 : 0);
+    const int i1 = tnx > 1 ? i0 + 1 : i0;
+    const int k1 = tnz > 1 ? k0 + 1 : k0;
+    T fu = u - T(i0), fw = w - T(k0);
+    fu = fu > T(0) ? fu : T(0);
+    fu = fu < T(1) ? fu : T(1);
+    fw = fw > T(0) ? fw : T(0);
+    fw = fw < T(1) ? fw : T(1);
+    if (tnx == 1) fu = T(0);
+    if (tnz == 1) fw = T(0);
+    const float* Hs = H + S.t_off;
+    const T h00 = (T)Hs[k0 * tnx + i0], h10 = (T)Hs[k0 * tnx + i1];
+    const T h01 = (T)Hs[k1 * tnx + i0], h11 = (T)Hs[k1 * tnx + i1];
+    const T gu = T(1) - fu, gw = T(1) - fw;
+    const T a0 = gu * h00 + fu * h10;
+    const T a1 = gu * h01 + fu * h11;
+    const T hs = gw * a0 + fw * a1;
+    const T dhu = gw * (h10 - h00) + fw * (h11 - h01);
+    const T dhw = gu * (h01 - h00) + fu * (h11 - h10);
+    const T gx = dhu * K.t_inv, gz = dhw * K.t_inv;
+    const T nn = (gx * gx + T(1)) + gz * gz;
+    const T inv = T(1) / d_sqrt(nn);
+    nrm[0] = -gx * inv;
+    nrm[1] = inv;
+    nrm[2] = -gz * inv;
+    return hs;
+}
+
+// Heightfield collision: projection onto the surface with the reference's
+// response term (fluid_system.h:344-346) while moving into the terrain.
+template <typename T>
+__device__ __forceinline__ void terrain_collide(const SceneConsts<T>& K, T hs, const T* nr, T* p, T* v)
+{
+    if (!(p[1] < hs)) return;
+    const T d = hs - p[1];
+    const T vn = (v[0] * nr[0] + v[1] * nr[1]) + v[2] * nr[2];
+    if (vn < T(0)) {
+        const T vl = d_sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+        const T s = T(1) + (T(0.5) * d) / (K.dt * vl);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = v[a] - (nr[a] * s) * vn;
+    }
+    p[1] = hs;
+}
+
+// Erosion / deposition exchange of one particle (build-defined).  Returns
+// the fixed-point amount taken from terrain cell *tc (negative: deposited),
+// or 0 with *tc = -1 outside the contact band.
+template <typename T>
+__device__ __forceinline__ int64_t erode_exchange(const SceneDev& S, const SceneConsts<T>& K, const T* p, T hs,
+                                                  const T* v, int64_t sed, int32_t* tc)
+{
+    *tc = -1;
+    const T gap = p[1] - hs;
+    if (!(gap <= K.band)) return 0;
+    const int ti = cidx<T>(d_floor((p[0] - K.t_org[0]) * K.t_inv), S.tnx - 1);
+    const int tk = cidx<T>(d_floor((p[2] - K.t_org[2]) * K.t_inv), S.tnz - 1);
+    *tc = tk * S.tnx + ti;
+    const T sp = d_sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    const T cap = K.kcv * sp;
+    const T s = (T)sed * K.dn;
+    T d;
+    if (cap > s) d = K.ke * (cap - s);
+    else d = -(K.kd * (s - cap));
+    if (!(d_fabs(d) <= K.dmax)) d = T(0);
+    int64_t dq = (int64_t)d_rint(d * K.up);
+    if (dq < -sed) dq = -sed;
+    return dq;
+}
+
+template <typename T, bool COMP>
+__global__ __launch_bounds__(512) void k_force(DevState D, int nblk, int remap)
+{
+    const int b = xcd_block(blockIdx.x, nblk, remap);
+    const int64_t slot = (int64_t)b * blockDim.x + threadIdx.x;
+    if (slot >= D.P) return;
+    const SceneDev& S = D.scenes[wave_scene(D.chunk_scene, slot)];
+    V4<T>* __restrict__ upos = (V4<T>*)D.u_pos;
+    V4<T>* __restrict__ uvel = (V4<T>*)D.u_vel;
+    const V4<T>* __restrict__ spos = (const V4<T>*)D.s_pos;
+    const V4<T>* __restrict__ svel = (const V4<T>*)D.s_vel;
+    if (slot >= S.p_begin + S.n) {
+        // padding: carried through, counted into the scene's pad cell
+        upos[slot] = spos[slot];
+        uvel[slot] = svel[slot];
+        D.u_sed[slot] = D.s_sed[slot];
+        D.u_id[slot] = D.s_id[slot];
+        const int64_t key = S.cell_off + S.ncell;
+        D.key[slot] = (int32_t)key;
+        D.arank[slot] = atomicAdd(&D.cnt[key], 1u);
+        if (S.erosion == 2) D.tcell[slot] = -1;
+        return;
+    }
+    const SceneConsts<T>& K = SceneSel<T>::get(S);
+    const V4<T> pi = spos[slot];
+    const V4<T> vi = svel[slot];
+    const T rho_i = ((const T*)D.rho)[slot];
+    const T a_i = ((const V2<T>*)D.aux)[slot].x;
+    ForceOut<T> o;
+    force_sums<T, COMP, false>(D, S, K, slot, pi, vi, rho_i, a_i, o);
+
+    // symplectic Euler (fluid_system.h:330-331)
+    T p[3], v[3];
+    const T xin[3] = {pi.x, pi.y, pi.z}, vin[3] = {vi.x, vi.y, vi.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        v[a] = vin[a] + o.acc[a] * K.dt;
+        p[a] = xin[a] + v[a] * K.dt;
+    }
+    int64_t sed = D.s_sed[slot];
+    const bool moving = K.dt != T(0);
+    if (moving) box_response<T>(K, p, v);
+    int32_t tc = -1;
+    int64_t dq = 0;
+    if (S.tnx > 0) {
+        T nr[3];
+        const T hs = terrain_sample<T>(S, K, D.height, p[0], p[2], nr);
+        if (moving) {
+            terrain_collide<T>(K, hs, nr, p, v);
+            if (S.erosion) dq = erode_exchange<T>(S, K, p, hs, v, sed, &tc);
+        }
+    }
+    if (S.erosion == 1 && dq != 0) {
+        sed += dq;
+        atomicAdd((unsigned long long*)&D.vol[S.t_off + tc], (unsigned long long)(-dq));
+    } else if (S.erosion == 2) {
+        sed += dq;
+        D.dq[slot] = dq;
+        D.tcell[slot] = dq != 0 ? S.t_off + tc : -1;
+    }
+    V4<T> np, nv;
+    np.x = p[0]; np.y = p[1]; np.z = p[2]; np.w = T(0);
+    nv.x = v[0]; nv.y = v[1]; nv.z = v[2]; nv.w = T(0);
+    upos[slot] = np;
+    uvel[slot] = nv;
+    D.u_sed[slot] = sed;
+    D.u_id[slot] = D.s_id[slot];
+    if (!(d_finite(p[0]) && d_finite(p[1]) && d_finite(p[2]) && d_finite(v[0]) && d_finite(v[1]) &&
+          d_finite(v[2])))
+        atomicOr(D.status, 1u);
+    // cell count of the new position for the next step's sort
+    int kx, ky, kz;
+    cell_of<T>(K, S.dims, p[0], p[1], p[2], kx, ky, kz);
+    const int64_t key = S.cell_off + ((int64_t)kz * S.dims[1] + ky) * S.dims[0] + kx;
+    D.key[slot] = (int32_t)key;
+    D.arank[slot] = atomicAdd(&D.cnt[key], 1u);
+}
+
+// --------------------------------------------------------------- terrain
+
+__global__ __launch_bounds__(256) void k_heights(DevState D)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.TC) return;
+    const SceneDev& S = D.scenes[wave_scene(D.tchunk_scene, t)];
+    if (t - S.t_off >= (int64_t)S.tnx * S.tnz) return;
+    const float v = (float)D.vol[t];
+    D.height[t] = S.t_y0 + v * S.hq;
+}
+
+// Two-phase erosion, phase 2a: count exchanges per terrain cell.
+__global__ __launch_bounds__(256) void k_tcount(DevState D)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D.P) return;
+    const SceneDev& S = D.scenes[wave_scene(D.chunk_scene, p)];
+    if (S.erosion != 2) return;
+    const int32_t tc = D.tcell[p];
+    if (tc >= 0) atomicAdd(&D.tcnt[tc], 1u);
+}
+
+// Phase 2b: place each exchange in its terrain cell's segment.
+__global__ __launch_bounds__(256) void k_tscatter(DevState D)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D.P) return;
+    const SceneDev& S = D.scenes[wave_scene(D.chunk_scene, p)];
+    if (S.erosion != 2) return;
+    const int32_t tc = D.tcell[p];
+    if (tc < 0) return;
+    const uint32_t at = D.tstart[tc] + atomicAdd(&D.tcnt[tc], 1u);
+    D.tdq[at] = D.dq[p];
+}
+
+// Phase 2c: one read-modify-write per terrain cell; the integer sum does not
+// depend on the order the segment was filled in.
+__global__ __launch_bounds__(256) void k_tapply(DevState D)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.TC) return;
+    const uint32_t s0 = D.tstart[t], s1 = D.tstart[t + 1];
+    int64_t sum = 0;
+    for (uint32_t j = s0; j < s1; ++j) sum += D.tdq[j];
+    if (s1 > s0) D.vol[t] -= sum;
+    D.tcnt[t] = 0u;
+}
+
+// ----------------------------------------------------------- inspection
+
+template <typename T, bool COMP>
+__global__ void k_inspect(DevState D, int64_t slot, double* out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const SceneDev& S = D.scenes[D.chunk_scene[slot >> 6]];
+    const SceneConsts<T>& K = SceneSel<T>::get(S);
+    const V4<T> pi = ((const V4<T>*)D.s_pos)[slot];
+    const V4<T> vi = ((const V4<T>*)D.s_vel)[slot];
+    const T rho_i = ((const T*)D.rho)[slot];
+    const T a_i = ((const V2<T>*)D.aux)[slot].x;
+    ForceOut<T> o;
+    force_sums<T, COMP, true>(D, S, K, slot, pi, vi, rho_i, a_i, o);
+    out[0] = (double)rho_i;
+    out[1] = (double)(K.k * (rho_i - K.rho0));
+    for (int a = 0; a < 3; ++a) {
+        out[2 + a] = (double)o.acc[a];
+        out[5 + a] = (double)o.pf[a];
+        out[8 + a] = (double)o.vf[a];
+        out[11 + a] = (double)o.gf[a];
+        out[14 + a] = (double)o.sf[a];
+        out[17 + a] = (double)o.nr[a];
+    }
+    out[20] = (double)o.nbmax;
+    out[21] = (double)o.count;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_neighbours(DevState D, int scene, const int64_t* __restrict__ off,
+                                                    int32_t* __restrict__ counts, int32_t* __restrict__ idx)
+{
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= D.P) return;
+    const int si = D.chunk_scene[slot >> 6];
+    if (si != scene) return;
+    const SceneDev& S = D.scenes[si];
+    if (slot >= S.p_begin + S.n) return;
+    const SceneConsts<T>& K = SceneSel<T>::get(S);
+    const V4<T>* pos = (const V4<T>*)D.s_pos;
+    const V4<T> pi = pos[slot];
+    int kx, ky, kz;
+    cell_of<T>(K, S.dims, pi.x, pi.y, pi.z, kx, ky, kz);
+    const int64_t local = slot - S.p_begin;
+    int32_t c = 0;
+    const int64_t o = idx ? off[local] : 0;
+    visit_neighbours<T>(S, K, D.start, pos, pi.x, pi.y, pi.z, kx, ky, kz,
+                        [&](uint32_t j, T, T, T, T) {
+                            if (idx) idx[o + c] = D.s_id[j];
+                            ++c;
+                        });
+    counts[local] = c;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_keys(DevState D, int scene, int32_t* __restrict__ keys)
+{
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= D.P) return;
+    const int si = D.chunk_scene[slot >> 6];
+    if (si != scene) return;
+    const SceneDev& S = D.scenes[si];
+    if (slot >= S.p_begin + S.n) return;
+    const SceneConsts<T>& K = SceneSel<T>::get(S);
+    const V4<T> p = ((const V4<T>*)D.u_pos)[slot];
+    int kx, ky, kz;
+    cell_of<T>(K, S.dims, p.x, p.y, p.z, kx, ky, kz);
+    keys[slot - S.p_begin] = (kz * S.dims[1] + ky) * S.dims[0] + kx;
+}
+
+__global__ void k_clear_status(uint32_t* status)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *status = 0u;
+}
+
+// Mass bookkeeping of one scene: FP64 sums of terrain volume and carried
+// sediment, and the exact mod-2^64 sum of their fixed-point units.  Each
+// block writes its partial; k_mass_final adds them in block order.
+__global__ __launch_bounds__(256) void k_mass_partial(DevState D, int scene, double* __restrict__ part)
+{
+    __shared__ double s_t[256];
+    __shared__ double s_s[256];
+    __shared__ unsigned long long s_u[256];
+    const SceneDev& S = D.scenes[scene];
+    const int64_t ncells = (int64_t)S.tnx * S.tnz;
+    const double unit = ldexp(1.0, -S.fx_exp);
+    double st = 0.0, ss = 0.0;
+    unsigned long long su = 0ull;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncells; t += stride) {
+        const int64_t v = D.vol[S.t_off + t];
+        st += (double)v * unit;
+        su += (unsigned long long)v;
+    }
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < S.n; p += stride) {
+        const int64_t v = D.u_sed[S.p_begin + p];
+        ss += (double)v * unit;
+        su += (unsigned long long)v;
+    }
+    s_t[threadIdx.x] = st;
+    s_s[threadIdx.x] = ss;
+    s_u[threadIdx.x] = su;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s_t[threadIdx.x] += s_t[threadIdx.x + o];
+            s_s[threadIdx.x] += s_s[threadIdx.x + o];
+            s_u[threadIdx.x] += s_u[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = s_t[0];
+        part[3 * blockIdx.x + 1] = s_s[0];
+        part[3 * blockIdx.x + 2] = __longlong_as_double((long long)s_u[0]);
+    }
+}
+
+__global__ void k_mass_final(const double* __restrict__ part, int nblocks, double* __restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double st = 0.0, ss = 0.0;
+    unsigned long long su = 0ull;
+    for (int b = 0; b < nblocks; ++b) {
+        st += part[3 * b];
+        ss += part[3 * b + 1];
+        su += (unsigned long long)__double_as_longlong(part[3 * b + 2]);
+    }
+    out[0] = st;
+    out[1] = ss;
+    out[2] = __longlong_as_double((long long)su);
+}
+
+inline unsigned grid_of(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+
+#define SPHE_PREC_SWITCH(prec, CALL_F32C, CALL_F32N, CALL_F64) \
+    do {                                                        \
+        if ((prec) == 0) { CALL_F32C; }                         \
+        else if ((prec) == 1) { CALL_F32N; }                    \
+        else { CALL_F64; }                                      \
+    } while (0)
+
+void sphe_launch_hash(const DevState& D, hipStream_t st)
+{
+    if (D.P == 0) return;
+    if (D.precision == 2) k_hash_count<double><<<grid_of(D.P, 256), 256, 0, st>>>(D);
+    else k_hash_count<float><<<grid_of(D.P, 256), 256, 0, st>>>(D);
+}
+
+void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, hipStream_t st)
+{
+    const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    k_scan_reduce<<<(unsigned)nb, 256, 0, st>>>(cnt, n, partial);
+    k_scan_partials<<<1, 256, 0, st>>>(partial, nb);
+    k_scan_down<<<(unsigned)nb, 256, 0, st>>>(cnt, start, n, partial);
+}
+
+int64_t sphe_scan_partials(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+void sphe_launch_scatter(const DevState& D, hipStream_t st)
+{
+    if (D.P == 0) return;
+    k_scatter<<<grid_of(D.P, 256), 256, 0, st>>>(D);
+}
+
+void sphe_launch_reorder(const DevState& D, hipStream_t st)
+{
+    if (D.P == 0) return;
+    if (D.precision == 2) k_reorder<double><<<grid_of(D.P, 256), 256, 0, st>>>(D);
+    else k_reorder<float><<<grid_of(D.P, 256), 256, 0, st>>>(D);
+}
+
+void sphe_launch_density(const DevState& D, int block, int remap, hipStream_t st)
+{
+    if (D.P == 0) return;
+    const int nblk = (int)grid_of(D.P, block);
+    SPHE_PREC_SWITCH(D.precision,
+                     (k_density<float, true><<<nblk, block, 0, st>>>(D, nblk, remap)),
+                     (k_density<float, false><<<nblk, block, 0, st>>>(D, nblk, remap)),
+                     (k_density<double, false><<<nblk, block, 0, st>>>(D, nblk, remap)));
+}
+
+void sphe_launch_force(const DevState& D, int block, int remap, hipStream_t st)
+{
+    if (D.P == 0) return;
+    const int nblk = (int)grid_of(D.P, block);
+    SPHE_PREC_SWITCH(D.precision,
+                     (k_force<float, true><<<nblk, block, 0, st>>>(D, nblk, remap)),
+                     (k_force<float, false><<<nblk, block, 0, st>>>(D, nblk, remap)),
+                     (k_force<double, false><<<nblk, block, 0, st>>>(D, nblk, remap)));
+}
+
+void sphe_launch_heights(const DevState& D, hipStream_t st)
+{
+    if (D.TC == 0) return;
+    k_heights<<<grid_of(D.TC, 256), 256, 0, st>>>(D);
+}
+
+void sphe_launch_two_phase(const DevState& D, hipStream_t st)
+{
+    if (D.TC == 0 || D.P == 0) return;
+    k_tcount<<<grid_of(D.P, 256), 256, 0, st>>>(D);
+    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, st);
+    k_tscatter<<<grid_of(D.P, 256), 256, 0, st>>>(D);
+    k_tapply<<<grid_of(D.TC, 256), 256, 0, st>>>(D);
+}
+
+void sphe_launch_clear_status(const DevState& D, hipStream_t st) { k_clear_status<<<1, 64, 0, st>>>(D.status); }
+
+void sphe_launch_inspect(const DevState& D, int64_t slot, double* out, hipStream_t st)
+{
+    SPHE_PREC_SWITCH(D.precision,
+                     (k_inspect<float, true><<<1, 64, 0, st>>>(D, slot, out)),
+                     (k_inspect<float, false><<<1, 64, 0, st>>>(D, slot, out)),
+                     (k_inspect<double, false><<<1, 64, 0, st>>>(D, slot, out)));
+}
+
+void sphe_launch_neighbours(const DevState& D, int scene, const int64_t* off, int32_t* counts, int32_t* idx,
+                            hipStream_t st)
+{
+    if (D.P == 0) return;
+    if (D.precision == 2) k_neighbours<double><<<grid_of(D.P, 256), 256, 0, st>>>(D, scene, off, counts, idx);
+    else k_neighbours<float><<<grid_of(D.P, 256), 256, 0, st>>>(D, scene, off, counts, idx);
+}
+
+void sphe_launch_keys(const DevState& D, int scene, int32_t* keys, hipStream_t st)
+{
+    if (D.P == 0) return;
+    if (D.precision == 2) k_keys<double><<<grid_of(D.P, 256), 256, 0, st>>>(D, scene, keys);
+    else k_keys<float><<<grid_of(D.P, 256), 256, 0, st>>>(D, scene, keys);
+}
+
+int sphe_mass_blocks() { return 256; }
+
+void sphe_launch_mass(const DevState& D, int scene, double* part, double* out, hipStream_t st)
+{
+    k_mass_partial<<<sphe_mass_blocks(), 256, 0, st>>>(D, scene, part);
+    k_mass_final<<<1, 64, 0, st>>>(part, sphe_mass_blocks(), out);
+}

@@ -1,0 +1,1648 @@
+// sphe_runtime.cpp -- host runtime and C ABI (include/sphe.h) of the MI355X
+// SPH-Erosion step.
+//
+// A context owns one HIP stream and the device buffers of all its scenes
+// (layout in sphe_device.h).  Host staging arrays keep each scene's particles
+// in ascending id order, as the reference's std::vector<FluidParticle> keeps
+// them in insertion order (Erosion/fluid_system.h:482); they are the source
+// of truth after a host-side edit (Initialize, AddParticles, Reset,
+// set_particles, terrain load) and are refreshed from the device on demand
+// after steps.  The device state is rebuilt from them whenever the particle
+// counts or the grid change.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/sphe.h"
+#include "sphe_device.h"
+#include "sphe_launch.h"
+
+namespace {
+
+const float REF_PI = 3.141592f;  // Erosion/sphere.h:8
+
+struct Scene {
+    sphe_params prm;
+    float origin[3] = {0, 0, 0};  // m_Origin (fluid_system.h:484)
+    int init_num = 0;             // fluid_system.h:474
+    int next_id = 0;              // fluid_system.h:479
+    // terrain
+    int tnx = 0, tnz = 0;
+    float t_org[3] = {0, 0, 0};
+    float t_cell = 1.0f;
+    int fx_exp = 60;
+    std::vector<int64_t> vol;  // host copy of the fixed-point volumes
+    // particles in ascending id order
+    std::vector<double> pos, vel;
+    std::vector<int32_t> ids;
+    std::vector<int64_t> sed;
+    // layout (valid while the context layout is)
+    int64_t p_begin = 0, p_cap = 0, cell_off = 0, ncell = 0, t_off = 0, tcap = 0;
+    int dims[3] = {0, 0, 0};
+};
+
+struct ProfRec {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct sphe_ctx {
+    int precision = SPHE_FP32;
+    int device = 0;
+    std::string err;
+    hipStream_t stream = nullptr;
+    std::vector<Scene> scenes;
+
+    bool layout_valid = false;
+    bool device_valid = false;  // device holds the latest state
+    bool host_valid = true;     // host staging holds the latest state
+    bool counted = false;       // key / arank / cnt prepared for the next scan
+    bool stepped = false;       // sorted buffers hold the last step's pre-step state
+
+    DevState D{};
+    std::vector<void*> allocs;
+    std::vector<SceneDev> table;  // host copy of the uploaded scene table
+    bool table_uploaded = false;
+
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::string graph_sig;
+
+    bool profiling = false;
+    std::vector<ProfRec> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, std::pair<double, int>> prof;
+
+    int block = 256;
+    int remap = 1;
+
+    double* d_mass_part = nullptr;
+    double* d_mass_out = nullptr;
+    double* d_inspect = nullptr;
+};
+
+namespace {
+
+int fail(sphe_ctx* c, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail((ctx), SPHE_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+size_t real_size(int precision) { return precision == SPHE_FP64 ? sizeof(double) : sizeof(float); }
+
+// ---------------------------------------------------------------- derive
+
+float pred_threshold(float h)
+{
+    // glm::length(r) <= h (fluid_system.h:118-119) <=> d2 <= T(h), the
+    // largest float whose correctly rounded square root is <= h.
+    float t = h * h;
+    while (std::sqrt(std::nextafter(t, INFINITY)) <= h) t = std::nextafter(t, INFINITY);
+    while (t > 0.0f && std::sqrt(t) > h) t = std::nextafter(t, 0.0f);
+    return t;
+}
+
+void grid_dims(const sphe_params& p, int dims[3])
+{
+    const float h = p.h;
+    const float cell = h * 1.00390625f;  // h (1 + 2^-8): 27 cells always cover the radius
+    const float inv = 1.0f / cell;
+    for (int a = 0; a < 3; ++a) {
+        const float lo = p.box_center[a] - p.box_half;
+        const float hi = p.box_center[a] + p.box_half;
+        dims[a] = (int)std::floor((hi - lo) * inv) + 1;
+    }
+}
+
+void derive(const Scene& s, SceneDev& d)
+{
+    const sphe_params& p = s.prm;
+    SceneConsts<float>& f = d.f;
+    const float h = p.h;
+    f.h = h;
+    f.h2 = h * h;
+    f.thr = pred_threshold(h);
+    f.t3h2 = 3.0f * f.h2;
+    // dist < 10e-5 (fluid_system.h:438): float dist against the double 1e-4,
+    // i.e. dist <= fl32(1e-4)
+    f.eps_r = std::nextafter(1e-4f, INFINITY);
+    f.inv_sqrt3 = 1.0f / std::sqrt(3.0f);
+    const double PI_D = (double)REF_PI;
+    const double hd = (double)h;
+    const double h6 = hd * hd * hd * hd * hd * hd, h9 = h6 * hd * hd * hd;
+    const double mass = (double)p.mass, visc = (double)p.visc;
+    f.mc6 = (float)(mass * (315.0 / (64.0 * PI_D * h9)));
+    f.pf_scale = (float)(mass * (45.0 / (PI_D * h6)));
+    f.vf_scale = (float)(visc * (45.0 / (PI_D * h6)));
+    f.ng = (float)(-(945.0 / (32.0 * PI_D * h9)));
+    f.rho0 = p.rest_density;
+    f.k = p.stiffness;
+    f.mass = p.mass;
+    f.surf_tens = p.surf_tens;
+    f.dt = p.dt;
+    const float cell = h * 1.00390625f;
+    f.inv_cell = 1.0f / cell;
+    for (int a = 0; a < 3; ++a) {
+        f.g[a] = p.gravity[a];
+        f.box_c[a] = p.box_center[a];
+        f.box_lo[a] = p.box_center[a] - p.box_half;
+        f.box_hi[a] = p.box_center[a] + p.box_half;
+        f.org[a] = f.box_lo[a];
+        f.t_org[a] = s.t_org[a];
+    }
+    grid_dims(p, d.dims);
+    f.t_inv = 1.0f / s.t_cell;
+    const double area = (double)s.t_cell * (double)s.t_cell;
+    const double kc = (double)p.kc, rho0 = (double)p.rest_density;
+    f.kcv = (float)(kc * mass / rho0);
+    f.ke = p.ke;
+    f.kd = p.kd;
+    f.band = p.band;
+    f.up = (float)std::ldexp(1.0, s.fx_exp);
+    f.dn = (float)std::ldexp(1.0, -s.fx_exp);
+    f.dmax = (float)std::ldexp(1.0, 44 - s.fx_exp);
+
+    // FP64 comparison path: the same formulas in double from the FP32 values
+    SceneConsts<double>& g = d.d;
+    g.h = hd;
+    g.h2 = hd * hd;
+    g.thr = hd * hd;
+    g.t3h2 = 3.0 * hd * hd;
+    g.eps_r = 1e-4;
+    g.inv_sqrt3 = 1.0 / std::sqrt(3.0);
+    g.mc6 = mass * (315.0 / (64.0 * PI_D * h9));
+    g.pf_scale = mass * (45.0 / (PI_D * h6));
+    g.vf_scale = visc * (45.0 / (PI_D * h6));
+    g.ng = -(945.0 / (32.0 * PI_D * h9));
+    g.rho0 = rho0;
+    g.k = (double)p.stiffness;
+    g.mass = mass;
+    g.surf_tens = (double)p.surf_tens;
+    g.dt = (double)p.dt;
+    g.inv_cell = 1.0 / (hd * (1.0 + std::ldexp(1.0, -8)));
+    for (int a = 0; a < 3; ++a) {
+        g.g[a] = (double)p.gravity[a];
+        g.box_c[a] = (double)f.box_c[a];
+        g.box_lo[a] = (double)f.box_lo[a];
+        g.box_hi[a] = (double)f.box_hi[a];
+        g.org[a] = (double)p.box_center[a] - (double)p.box_half;
+        g.t_org[a] = (double)s.t_org[a];
+    }
+    g.t_inv = 1.0 / (double)s.t_cell;
+    g.kcv = kc * mass / rho0;
+    g.ke = (double)p.ke;
+    g.kd = (double)p.kd;
+    g.band = (double)p.band;
+    g.up = std::ldexp(1.0, s.fx_exp);
+    g.dn = std::ldexp(1.0, -s.fx_exp);
+    g.dmax = std::ldexp(1.0, 44 - s.fx_exp);
+
+    d.hq = (float)(std::ldexp(1.0, -s.fx_exp) / area);
+    d.t_y0 = s.t_org[1];
+    d.tnx = s.tnx;
+    d.tnz = s.tnz;
+    d.fx_exp = s.fx_exp;
+    d.erosion = (s.tnx > 0 && s.tnz > 0) ? p.erosion : 0;
+    d.p_begin = (int32_t)s.p_begin;
+    d.p_cap = (int32_t)s.p_cap;
+    d.n = (int32_t)s.ids.size();
+    d.tcap = (int32_t)s.tcap;
+    d.cell_off = (int32_t)s.cell_off;
+    d.ncell = (int32_t)s.ncell;
+    d.t_off = (int32_t)s.t_off;
+}
+
+// ---------------------------------------------------------------- lattice
+
+// FluidSystemSPH::Initialize / AddParticles lattice (fluid_system.h:80-95,
+// 234-249): i, j, k < cbrt(n); coordinates computed in double, narrowed to
+// float, origin added in float.
+void append_lattice(Scene& s, int n)
+{
+    const double cb = std::cbrt((double)n);
+    int side = 0;
+    while ((double)side < cb) ++side;
+    for (int i = 0; i < side; ++i)
+        for (int j = 0; j < side; ++j)
+            for (int k = 0; k < side; ++k) {
+                const float x = (float)(-0.2 + i * 0.025);
+                const float y = (float)(-0.05 + j * 0.025);
+                const float z = (float)(-0.15 + k * 0.025);
+                s.pos.push_back((double)(x + s.origin[0]));
+                s.pos.push_back((double)(y + s.origin[1]));
+                s.pos.push_back((double)(z + s.origin[2]));
+                for (int a = 0; a < 3; ++a) s.vel.push_back(0.0);
+                s.ids.push_back(s.next_id++);
+                s.sed.push_back(0);
+            }
+}
+
+// keeps each scene's staging arrays in ascending id order (stable)
+void sort_by_id(Scene& s)
+{
+    const size_t n = s.ids.size();
+    bool sorted = true;
+    for (size_t i = 1; i < n && sorted; ++i) sorted = s.ids[i - 1] <= s.ids[i];
+    if (sorted) return;
+    std::vector<size_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) { return s.ids[a] < s.ids[b]; });
+    std::vector<double> p(3 * n), v(3 * n);
+    std::vector<int32_t> id(n);
+    std::vector<int64_t> sd(n);
+    for (size_t t = 0; t < n; ++t) {
+        const size_t i = perm[t];
+        for (int a = 0; a < 3; ++a) {
+            p[3 * t + a] = s.pos[3 * i + a];
+            v[3 * t + a] = s.vel[3 * i + a];
+        }
+        id[t] = s.ids[i];
+        sd[t] = s.sed[i];
+    }
+    s.pos.swap(p);
+    s.vel.swap(v);
+    s.ids.swap(id);
+    s.sed.swap(sd);
+}
+
+// ------------------------------------------------------------- buffers
+
+void free_device(sphe_ctx* c)
+{
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    c->graph_sig.clear();
+    for (void* p : c->allocs) (void)hipFree(p);
+    c->allocs.clear();
+    c->D = DevState{};
+    c->layout_valid = false;
+    c->device_valid = false;
+    c->counted = false;
+    c->stepped = false;
+    c->table_uploaded = false;
+}
+
+int dalloc(sphe_ctx* c, void** p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(c, SPHE_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    c->allocs.push_back(*p);
+    return SPHE_OK;
+}
+
+#define DALLOC(ptr, bytes)                                        \
+    do {                                                          \
+        void* p_;                                                 \
+        int r_ = dalloc(c, &p_, (bytes));                         \
+        if (r_) return r_;                                        \
+        ptr = (decltype(ptr))p_;                                  \
+    } while (0)
+
+int build_layout(sphe_ctx* c)
+{
+    free_device(c);
+    int64_t P = 0, C = 0, TC = 0;
+    for (Scene& s : c->scenes) {
+        s.p_begin = P;
+        s.p_cap = round_up((int64_t)s.ids.size(), SPHE_WAVE);
+        P += s.p_cap;
+        grid_dims(s.prm, s.dims);
+        s.ncell = (int64_t)s.dims[0] * s.dims[1] * s.dims[2];
+        s.cell_off = C;
+        C += s.ncell + 1;
+        s.t_off = TC;
+        s.tcap = round_up((int64_t)s.tnx * s.tnz, SPHE_WAVE);
+        TC += s.tcap;
+    }
+    if (P > INT32_MAX / 2 || C > INT32_MAX / 2 || TC > INT32_MAX / 2)
+        return fail(c, SPHE_ERANGE, "context too large (slots %lld, cells %lld)", (long long)P, (long long)C);
+    DevState& D = c->D;
+    D.P = P;
+    D.C = C;
+    D.TC = TC;
+    D.nscenes = (int)c->scenes.size();
+    D.precision = c->precision;
+    const size_t E = real_size(c->precision);
+    const size_t Pn = (size_t)P;
+    DALLOC(D.u_pos, Pn * 4 * E);
+    DALLOC(D.u_vel, Pn * 4 * E);
+    DALLOC(D.s_pos, Pn * 4 * E);
+    DALLOC(D.s_vel, Pn * 4 * E);
+    DALLOC(D.u_sed, Pn * 8);
+    DALLOC(D.s_sed, Pn * 8);
+    DALLOC(D.u_id, Pn * 4);
+    DALLOC(D.s_id, Pn * 4);
+    DALLOC(D.rho, Pn * E);
+    DALLOC(D.aux, Pn * 2 * E);
+    DALLOC(D.key, Pn * 4);
+    DALLOC(D.arank, Pn * 4);
+    DALLOC(D.tmp_id, Pn * 4);
+    DALLOC(D.tmp_key, Pn * 4);
+    DALLOC(D.tmp_src, Pn * 4);
+    DALLOC(D.cnt, (size_t)(C + 1) * 4);
+    DALLOC(D.start, (size_t)(C + 1) * 4);
+    D.npartial = sphe_scan_partials(C + 1);
+    DALLOC(D.partial, (size_t)D.npartial * 4);
+    DALLOC(D.vol, (size_t)TC * 8);
+    DALLOC(D.height, (size_t)TC * 4);
+    DALLOC(D.dq, Pn * 8);
+    DALLOC(D.tcell, Pn * 4);
+    DALLOC(D.tcnt, (size_t)(TC + 1) * 4);
+    DALLOC(D.tstart, (size_t)(TC + 1) * 4);
+    D.ntpartial = sphe_scan_partials(TC + 1);
+    DALLOC(D.tpartial, (size_t)D.ntpartial * 4);
+    DALLOC(D.tdq, Pn * 8);
+    DALLOC(D.status, 64);
+    DALLOC(c->d_mass_part, (size_t)sphe_mass_blocks() * 3 * 8);
+    DALLOC(c->d_mass_out, 64);
+    DALLOC(c->d_inspect, 64 * 8);
+    SceneDev* sc;
+    DALLOC(sc, sizeof(SceneDev) * std::max<size_t>(1, c->scenes.size()));
+    D.scenes = sc;
+    int32_t* cs;
+    DALLOC(cs, (size_t)(P / SPHE_WAVE + 1) * 4);
+    D.chunk_scene = cs;
+    int32_t* tcs;
+    DALLOC(tcs, (size_t)(TC / SPHE_WAVE + 1) * 4);
+    D.tchunk_scene = tcs;
+
+    std::vector<int32_t> hcs((size_t)(P / SPHE_WAVE + 1), 0), htcs((size_t)(TC / SPHE_WAVE + 1), 0);
+    for (size_t si = 0; si < c->scenes.size(); ++si) {
+        const Scene& s = c->scenes[si];
+        for (int64_t k = s.p_begin / SPHE_WAVE; k < (s.p_begin + s.p_cap) / SPHE_WAVE; ++k) hcs[k] = (int32_t)si;
+        for (int64_t k = s.t_off / SPHE_WAVE; k < (s.t_off + s.tcap) / SPHE_WAVE; ++k) htcs[k] = (int32_t)si;
+    }
+    HIPCHK(c, hipMemcpyAsync(cs, hcs.data(), hcs.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(tcs, htcs.data(), htcs.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(D.cnt, 0, (size_t)(C + 1) * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(D.tcnt, 0, (size_t)(TC + 1) * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(D.status, 0, 64, c->stream));
+    if (TC > 0) HIPCHK(c, hipMemsetAsync(D.vol, 0, (size_t)TC * 8, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->layout_valid = true;
+    c->device_valid = false;
+    return SPHE_OK;
+}
+
+template <typename T>
+void pack_particles(const Scene& s, int64_t cap, std::vector<T>& pos4, std::vector<T>& vel4,
+                    std::vector<int64_t>& sed, std::vector<int32_t>& ids)
+{
+    const int64_t n = (int64_t)s.ids.size();
+    pos4.assign((size_t)cap * 4, T(0));
+    vel4.assign((size_t)cap * 4, T(0));
+    sed.assign((size_t)cap, 0);
+    ids.assign((size_t)cap, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            pos4[4 * i + a] = (T)s.pos[3 * i + a];
+            vel4[4 * i + a] = (T)s.vel[3 * i + a];
+        }
+        sed[i] = s.sed[i];
+        ids[i] = s.ids[i];
+    }
+    // padding: unique ids above the scene's largest, parked at the box centre
+    int32_t top = 0;
+    for (int64_t i = 0; i < n; ++i) top = std::max(top, s.ids[i]);
+    for (int64_t i = n; i < cap; ++i) {
+        for (int a = 0; a < 3; ++a) pos4[4 * i + a] = (T)s.prm.box_center[a];
+        ids[i] = top + 1 + (int32_t)(i - n);
+    }
+}
+
+template <typename T>
+int upload_t(sphe_ctx* c)
+{
+    DevState& D = c->D;
+    for (const Scene& s : c->scenes) {
+        if (s.p_cap == 0) continue;
+        std::vector<T> p4, v4;
+        std::vector<int64_t> sd;
+        std::vector<int32_t> id;
+        pack_particles<T>(s, s.p_cap, p4, v4, sd, id);
+        const size_t off = (size_t)s.p_begin;
+        HIPCHK(c, hipMemcpy((T*)D.u_pos + 4 * off, p4.data(), p4.size() * sizeof(T), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy((T*)D.u_vel + 4 * off, v4.data(), v4.size() * sizeof(T), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(D.u_sed + off, sd.data(), sd.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(D.u_id + off, id.data(), id.size() * 4, hipMemcpyHostToDevice));
+    }
+    return SPHE_OK;
+}
+
+int upload_table(sphe_ctx* c, bool force);
+
+int upload(sphe_ctx* c)
+{
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int r = c->precision == SPHE_FP64 ? upload_t<double>(c) : upload_t<float>(c);
+    if (r) return r;
+    DevState& D = c->D;
+    for (const Scene& s : c->scenes) {
+        const size_t nt = (size_t)s.tnx * s.tnz;
+        if (nt == 0) continue;
+        HIPCHK(c, hipMemcpy(D.vol + s.t_off, s.vol.data(), nt * 8, hipMemcpyHostToDevice));
+    }
+    r = upload_table(c, true);
+    if (r) return r;
+    sphe_launch_heights(D, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->device_valid = true;
+    c->counted = false;
+    c->stepped = false;
+    return SPHE_OK;
+}
+
+template <typename T>
+int download_t(sphe_ctx* c)
+{
+    DevState& D = c->D;
+    for (Scene& s : c->scenes) {
+        const size_t n = s.ids.size();
+        if (n == 0) continue;
+        std::vector<T> p4(4 * n), v4(4 * n);
+        std::vector<int64_t> sd(n);
+        std::vector<int32_t> id(n);
+        const size_t off = (size_t)s.p_begin;
+        HIPCHK(c, hipMemcpy(p4.data(), (T*)D.u_pos + 4 * off, p4.size() * sizeof(T), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(v4.data(), (T*)D.u_vel + 4 * off, v4.size() * sizeof(T), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(sd.data(), D.u_sed + off, n * 8, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(id.data(), D.u_id + off, n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            for (int a = 0; a < 3; ++a) {
+                s.pos[3 * i + a] = (double)p4[4 * i + a];
+                s.vel[3 * i + a] = (double)v4[4 * i + a];
+            }
+            s.ids[i] = id[i];
+            s.sed[i] = sd[i];
+        }
+        sort_by_id(s);
+    }
+    return SPHE_OK;
+}
+
+int sync_to_host(sphe_ctx* c)
+{
+    if (c->host_valid) return SPHE_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int r = c->precision == SPHE_FP64 ? download_t<double>(c) : download_t<float>(c);
+    if (r) return r;
+    for (Scene& s : c->scenes) {
+        const size_t nt = (size_t)s.tnx * s.tnz;
+        if (nt == 0) continue;
+        HIPCHK(c, hipMemcpy(s.vol.data(), c->D.vol + s.t_off, nt * 8, hipMemcpyDeviceToHost));
+    }
+    c->host_valid = true;
+    return SPHE_OK;
+}
+
+// Called before a host-side edit of particles, terrain or the grid.
+int begin_host_edit(sphe_ctx* c)
+{
+    int r = sync_to_host(c);
+    if (r) return r;
+    c->device_valid = false;
+    c->stepped = false;
+    c->layout_valid = false;
+    return SPHE_OK;
+}
+
+int ensure_device(sphe_ctx* c)
+{
+    if (!c->layout_valid) {
+        int r = build_layout(c);
+        if (r) return r;
+    }
+    if (!c->device_valid) {
+        int r = upload(c);
+        if (r) return r;
+    }
+    return SPHE_OK;
+}
+
+int upload_table(sphe_ctx* c, bool force)
+{
+    std::vector<SceneDev> t(c->scenes.size());
+    for (size_t i = 0; i < c->scenes.size(); ++i) {
+        std::memset(&t[i], 0, sizeof(SceneDev));
+        derive(c->scenes[i], t[i]);
+    }
+    if (!force && c->table_uploaded && t.size() == c->table.size() &&
+        std::memcmp(t.data(), c->table.data(), t.size() * sizeof(SceneDev)) == 0)
+        return SPHE_OK;
+    c->table = t;
+    if (!t.empty())
+        HIPCHK(c, hipMemcpyAsync((void*)c->D.scenes, c->table.data(), t.size() * sizeof(SceneDev),
+                                 hipMemcpyHostToDevice, c->stream));
+    c->table_uploaded = true;
+    return SPHE_OK;
+}
+
+// The grid of a scene follows h and the box; a change of either after the
+// layout was built needs a new layout.
+int check_grid(sphe_ctx* c)
+{
+    for (Scene& s : c->scenes) {
+        int dims[3];
+        grid_dims(s.prm, dims);
+        if (dims[0] != s.dims[0] || dims[1] != s.dims[1] || dims[2] != s.dims[2]) {
+            int r = begin_host_edit(c);
+            if (r) return r;
+            return ensure_device(c);
+        }
+    }
+    return SPHE_OK;
+}
+
+int prepare(sphe_ctx* c)
+{
+    int r = ensure_device(c);
+    if (r) return r;
+    r = check_grid(c);
+    if (r) return r;
+    return upload_table(c, false);
+}
+
+// ---------------------------------------------------------------- steps
+
+hipEvent_t get_event(sphe_ctx* c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+struct Timed {
+    sphe_ctx* c;
+    const char* name;
+    hipEvent_t a = nullptr;
+    Timed(sphe_ctx* ctx, const char* n) : c(ctx), name(n)
+    {
+        if (c->profiling) {
+            a = get_event(c);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Timed()
+    {
+        if (c->profiling) {
+            hipEvent_t b = get_event(c);
+            (void)hipEventRecord(b, c->stream);
+            c->pending.push_back(ProfRec{name, a, b});
+        }
+    }
+};
+
+bool any_erosion(const sphe_ctx* c, int mode)
+{
+    for (const Scene& s : c->scenes)
+        if (s.tnx > 0 && s.tnz > 0 && (mode < 0 ? s.prm.erosion != 0 : s.prm.erosion == mode)) return true;
+    return false;
+}
+
+void launch_sort(sphe_ctx* c)
+{
+    const DevState& D = c->D;
+    {
+        Timed t(c, "scan");
+        sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, c->stream);
+    }
+    {
+        Timed t(c, "scatter");
+        sphe_launch_scatter(D, c->stream);
+    }
+    {
+        Timed t(c, "reorder");
+        sphe_launch_reorder(D, c->stream);
+    }
+}
+
+void launch_step(sphe_ctx* c)
+{
+    const DevState& D = c->D;
+    launch_sort(c);
+    {
+        Timed t(c, "density");
+        sphe_launch_density(D, c->block, c->remap, c->stream);
+    }
+    {
+        Timed t(c, "force");
+        sphe_launch_force(D, c->block, c->remap, c->stream);
+    }
+    if (any_erosion(c, SPHE_EROSION_TWO_PHASE)) {
+        Timed t(c, "two_phase");
+        sphe_launch_two_phase(D, c->stream);
+    }
+    if (any_erosion(c, -1)) {
+        Timed t(c, "heights");
+        sphe_launch_heights(D, c->stream);
+    }
+}
+
+std::string plan_signature(const sphe_ctx* c)
+{
+    char b[256];
+    snprintf(b, sizeof(b), "%d/%d/%d/%lld/%lld/%lld/%d/%d/%p", c->precision, c->block, c->remap,
+             (long long)c->D.P, (long long)c->D.C, (long long)c->D.TC, (int)any_erosion(c, -1),
+             (int)any_erosion(c, SPHE_EROSION_TWO_PHASE), (const void*)c->D.u_pos);
+    return b;
+}
+
+int ensure_graph(sphe_ctx* c)
+{
+    const std::string sig = plan_signature(c);
+    if (c->gexec && sig == c->graph_sig) return SPHE_OK;
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    launch_step(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess) return fail(c, SPHE_EHIP, "graph capture: %s", hipGetErrorString(e));
+    c->graph = g;
+    HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+    c->graph_sig = sig;
+    return SPHE_OK;
+}
+
+// Counts the cell of every unsorted slot from cleared counters.  After a
+// re-upload the counters still hold the arrivals the last force pass made,
+// so they must be cleared before the state is counted again.
+int recount(sphe_ctx* c)
+{
+    HIPCHK(c, hipMemsetAsync(c->D.cnt, 0, (size_t)(c->D.C + 1) * 4, c->stream));
+    {
+        Timed t(c, "hash");
+        sphe_launch_hash(c->D, c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    c->counted = true;
+    return SPHE_OK;
+}
+
+int do_steps(sphe_ctx* c, int nsteps)
+{
+    int r = prepare(c);
+    if (r) return r;
+    if (c->D.P == 0) return SPHE_OK;
+    sphe_launch_clear_status(c->D, c->stream);
+    if (!c->counted) {
+        r = recount(c);
+        if (r) return r;
+    }
+    HIPCHK(c, hipGetLastError());
+    const bool graph = c->use_graph && !c->profiling;
+    if (graph) {
+        r = ensure_graph(c);
+        if (r) return r;
+    }
+    for (int k = 0; k < nsteps; ++k) {
+        if (graph) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+        else launch_step(c);
+    }
+    HIPCHK(c, hipGetLastError());
+    c->host_valid = false;
+    c->stepped = true;
+    return SPHE_OK;
+}
+
+int collect_profile(sphe_ctx* c)
+{
+    if (c->pending.empty()) return SPHE_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (ProfRec& p : c->pending) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, p.a, p.b));
+        auto& e = c->prof[p.name];
+        e.first += ms;
+        e.second += 1;
+        c->event_pool.push_back(p.a);
+        c->event_pool.push_back(p.b);
+    }
+    c->pending.clear();
+    return SPHE_OK;
+}
+
+// Sorts the current state without stepping (verification hooks).
+int sort_current(sphe_ctx* c)
+{
+    int r = prepare(c);
+    if (r) return r;
+    if (c->D.P == 0) return SPHE_OK;
+    if (!c->counted) {
+        r = recount(c);
+        if (r) return r;
+    }
+    launch_sort(c);
+    sphe_launch_density(c->D, c->block, c->remap, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->counted = false;
+    c->stepped = false;
+    // the sorted buffers now hold the current state; keep the unsorted
+    // state authoritative for the next step (its counts were consumed)
+    return SPHE_OK;
+}
+
+Scene* scene_of(sphe_ctx* c, int scene)
+{
+    if (!c || scene < 0 || scene >= (int)c->scenes.size()) return nullptr;
+    return &c->scenes[(size_t)scene];
+}
+
+const Scene* scene_of(const sphe_ctx* c, int scene)
+{
+    if (!c || scene < 0 || scene >= (int)c->scenes.size()) return nullptr;
+    return &c->scenes[(size_t)scene];
+}
+
+#define SCENE_OR_FAIL(var, ctx, idx)                                                   \
+    Scene* var = scene_of((ctx), (idx));                                               \
+    if (!var) return fail((ctx), SPHE_ERANGE, "scene %d out of range", (int)(idx))
+
+// Fixed-point unit 2^-fx_exp of the scene's volumes: chosen so that twice the
+// tallest column the box allows stays below 2^56 units.
+int set_terrain_heights(sphe_ctx* c, Scene& s, const float* heights)
+{
+    const size_t nt = (size_t)s.tnx * s.tnz;
+    double top = (double)s.prm.box_center[1] + (double)s.prm.box_half;
+    if (heights)
+        for (size_t t = 0; t < nt; ++t) top = std::max(top, (double)heights[t]);
+    const double col = std::max(1e-30, top - (double)s.t_org[1] + (double)s.t_cell);
+    const double vmax = col * 2.0 * (double)s.t_cell * (double)s.t_cell;
+    s.fx_exp = std::max(-60, std::min(100, 56 - (int)std::ceil(std::log2(vmax))));
+    s.vol.assign(nt, 0);
+    const double area = (double)s.t_cell * (double)s.t_cell;
+    const double up = std::ldexp(1.0, s.fx_exp);
+    for (size_t t = 0; t < nt; ++t) {
+        const double hgt = heights ? (double)heights[t] : (double)s.t_org[1];
+        if (!std::isfinite(hgt)) return fail(c, SPHE_EARG, "non-finite terrain height at cell %zu", t);
+        const double v = std::max(0.0, hgt - (double)s.t_org[1]) * area * up;
+        s.vol[t] = (int64_t)std::llround(v);
+    }
+    return SPHE_OK;
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+
+extern "C" {
+
+const char* sphe_version(void) { return "sphe 0.1.0 (gfx950)"; }
+
+void sphe_params_default(sphe_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    // Erosion/fluid_system.h:460-478
+    p->gravity[0] = 0.0f;
+    p->gravity[1] = -9.82f;
+    p->gravity[2] = 0.0f;
+    p->dt = 0.0f;
+    p->rest_density = 998.29f;
+    p->mass = 0.02f;
+    p->visc = 3.5f;
+    p->surf_tens = 0.0728f;
+    p->stiffness = 3.0f;
+    p->h = 0.0457f;
+    p->box_half = 0.2f;
+    p->erosion = SPHE_EROSION_OFF;
+    p->kc = 0.05f;
+    p->ke = 0.3f;
+    p->kd = 0.3f;
+    p->band = 0.0457f;
+}
+
+int sphe_create(int precision, sphe_ctx** out)
+{
+    if (!out) return SPHE_EARG;
+    *out = nullptr;
+    if (precision < SPHE_FP32 || precision > SPHE_FP64) return SPHE_EARG;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return SPHE_EHIP;
+    sphe_ctx* c = new sphe_ctx();
+    c->precision = precision;
+    c->device = dev;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return SPHE_EHIP;
+    }
+    *out = c;
+    return SPHE_OK;
+}
+
+int sphe_destroy(sphe_ctx* c)
+{
+    if (!c) return SPHE_EARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (ProfRec& p : c->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+    free_device(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SPHE_OK;
+}
+
+const char* sphe_last_error(const sphe_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int sphe_add_scene(sphe_ctx* c, const sphe_params* p, const sphe_terrain_desc* t, int* scene)
+{
+    if (!c || !p) return fail(c, SPHE_EARG, "null argument");
+    if (!(p->h > 0.0f) || !(p->box_half > 0.0f)) return fail(c, SPHE_EARG, "h and box_half must be > 0");
+    int dims[3];
+    grid_dims(*p, dims);
+    if ((double)dims[0] * dims[1] * dims[2] > 4.0e8) return fail(c, SPHE_ERANGE, "grid too fine for the box");
+    int r = begin_host_edit(c);
+    if (r) return r;
+    Scene s;
+    s.prm = *p;
+    if (t && t->nx > 0 && t->nz > 0) {
+        if (!(t->cell > 0.0f)) return fail(c, SPHE_EARG, "terrain cell must be > 0");
+        s.tnx = t->nx;
+        s.tnz = t->nz;
+        for (int a = 0; a < 3; ++a) s.t_org[a] = t->origin[a];
+        s.t_cell = t->cell;
+        r = set_terrain_heights(c, s, t->heights);
+        if (r) return r;
+    }
+    c->scenes.push_back(std::move(s));
+    if (scene) *scene = (int)c->scenes.size() - 1;
+    return SPHE_OK;
+}
+
+int sphe_scene_count(const sphe_ctx* c, int* count)
+{
+    if (!c || !count) return SPHE_EARG;
+    *count = (int)c->scenes.size();
+    return SPHE_OK;
+}
+
+int sphe_set_params(sphe_ctx* c, int scene, const sphe_params* p)
+{
+    if (!c || !p) return fail(c, SPHE_EARG, "null argument");
+    SCENE_OR_FAIL(s, c, scene);
+    if (!(p->h > 0.0f) || !(p->box_half > 0.0f)) return fail(c, SPHE_EARG, "h and box_half must be > 0");
+    s->prm = *p;
+    return SPHE_OK;
+}
+
+int sphe_get_params(const sphe_ctx* c, int scene, sphe_params* p)
+{
+    const Scene* s = scene_of(c, scene);
+    if (!s || !p) return SPHE_EARG;
+    *p = s->prm;
+    return SPHE_OK;
+}
+
+float* sphe_param_ptr(sphe_ctx* c, int scene, int which)
+{
+    Scene* s = scene_of(c, scene);
+    if (!s) return nullptr;
+    switch (which) {
+    case SPHE_P_MASS: return &s->prm.mass;
+    case SPHE_P_VISC: return &s->prm.visc;
+    case SPHE_P_SURF_TEN: return &s->prm.surf_tens;
+    case SPHE_P_P0: return &s->prm.rest_density;
+    case SPHE_P_GRAV: return s->prm.gravity;
+    default: return nullptr;
+    }
+}
+
+int sphe_set_dt(sphe_ctx* c, int scene, float dt)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    s->prm.dt = dt;
+    return SPHE_OK;
+}
+
+int sphe_get_dt(const sphe_ctx* c, int scene, float* dt)
+{
+    const Scene* s = scene_of(c, scene);
+    if (!s || !dt) return SPHE_EARG;
+    *dt = s->prm.dt;
+    return SPHE_OK;
+}
+
+int sphe_set_origin(sphe_ctx* c, int scene, const float o[3])
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (!o) return fail(c, SPHE_EARG, "null origin");
+    for (int a = 0; a < 3; ++a) s->origin[a] = o[a];
+    return SPHE_OK;
+}
+
+int sphe_get_origin(const sphe_ctx* c, int scene, float o[3])
+{
+    const Scene* s = scene_of(c, scene);
+    if (!s || !o) return SPHE_EARG;
+    for (int a = 0; a < 3; ++a) o[a] = s->origin[a];
+    return SPHE_OK;
+}
+
+int sphe_init_lattice(sphe_ctx* c, int scene, int n)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (n < 0) return fail(c, SPHE_EARG, "negative particle count");
+    int r = begin_host_edit(c);
+    if (r) return r;
+    // Initialize (fluid_system.h:74-102): the id counter restarts at 0 and
+    // the lattice is appended to whatever the scene holds.
+    s->init_num = n;
+    s->next_id = 0;
+    append_lattice(*s, n);
+    sort_by_id(*s);
+    return SPHE_OK;
+}
+
+int sphe_add_particles(sphe_ctx* c, int scene, int n)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (n < 0) return fail(c, SPHE_EARG, "negative particle count");
+    int r = begin_host_edit(c);
+    if (r) return r;
+    append_lattice(*s, n);
+    sort_by_id(*s);
+    return SPHE_OK;
+}
+
+int sphe_reset(sphe_ctx* c, int scene)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    int r = begin_host_edit(c);
+    if (r) return r;
+    // Reset (fluid_system.h:253-259)
+    s->pos.clear();
+    s->vel.clear();
+    s->ids.clear();
+    s->sed.clear();
+    s->next_id = 0;
+    append_lattice(*s, s->init_num);
+    return SPHE_OK;
+}
+
+static int set_particles_any(sphe_ctx* c, int scene, int n, const float* pf, const double* pd, const float* vf,
+                             const double* vd, const int32_t* ids, const int64_t* sed)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (n < 0 || (n > 0 && !pf && !pd)) return fail(c, SPHE_EARG, "bad particle arrays");
+    if (ids)
+        for (int i = 0; i < n; ++i)
+            if (ids[i] < 0) return fail(c, SPHE_EARG, "negative particle id at %d", i);
+    if (sed)
+        for (int i = 0; i < n; ++i)
+            if (sed[i] < 0) return fail(c, SPHE_EARG, "negative sediment at %d", i);
+    int r = begin_host_edit(c);
+    if (r) return r;
+    s->pos.resize((size_t)3 * n);
+    s->vel.assign((size_t)3 * n, 0.0);
+    s->ids.resize((size_t)n);
+    s->sed.assign((size_t)n, 0);
+    for (int i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            s->pos[3 * i + a] = pf ? (double)pf[3 * i + a] : pd[3 * i + a];
+            if (vf) s->vel[3 * i + a] = (double)vf[3 * i + a];
+            if (vd) s->vel[3 * i + a] = vd[3 * i + a];
+        }
+        s->ids[i] = ids ? ids[i] : i;
+        if (sed) s->sed[i] = sed[i];
+    }
+    int32_t top = -1;
+    for (int i = 0; i < n; ++i) top = std::max(top, s->ids[i]);
+    s->next_id = top + 1;
+    sort_by_id(*s);
+    return SPHE_OK;
+}
+
+int sphe_set_particles(sphe_ctx* c, int scene, int n, const float* pos3, const float* vel3, const int32_t* ids,
+                       const int64_t* sed_fx)
+{
+    return set_particles_any(c, scene, n, pos3, nullptr, vel3, nullptr, ids, sed_fx);
+}
+
+int sphe_set_particles_f64(sphe_ctx* c, int scene, int n, const double* pos3, const double* vel3,
+                           const int32_t* ids, const int64_t* sed_fx)
+{
+    return set_particles_any(c, scene, n, nullptr, pos3, nullptr, vel3, ids, sed_fx);
+}
+
+int sphe_num_particles(const sphe_ctx* c, int scene, int* n)
+{
+    const Scene* s = scene_of(c, scene);
+    if (!s || !n) return SPHE_EARG;
+    *n = (int)s->ids.size();
+    return SPHE_OK;
+}
+
+// density / pressure of the last step, in id order
+static int read_rho(sphe_ctx* c, Scene& s, std::vector<double>& rho)
+{
+    const size_t n = s.ids.size();
+    rho.assign(n, 0.0);
+    if (!c->stepped || n == 0) return SPHE_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<int32_t> id(n);
+    HIPCHK(c, hipMemcpy(id.data(), c->D.s_id + s.p_begin, n * 4, hipMemcpyDeviceToHost));
+    std::vector<double> r(n);
+    if (c->precision == SPHE_FP64) {
+        HIPCHK(c, hipMemcpy(r.data(), (double*)c->D.rho + s.p_begin, n * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> rf(n);
+        HIPCHK(c, hipMemcpy(rf.data(), (float*)c->D.rho + s.p_begin, n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) r[i] = rf[i];
+    }
+    std::vector<size_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) { return id[a] < id[b]; });
+    for (size_t t = 0; t < n; ++t) rho[t] = r[perm[t]];
+    return SPHE_OK;
+}
+
+int sphe_read_buffer(sphe_ctx* c, int scene, int which, void* out, size_t bytes)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    int r = sync_to_host(c);
+    if (r) return r;
+    const size_t n = s->ids.size();
+    const size_t comps = (which == SPHE_BUF_POS || which == SPHE_BUF_VEL) ? 3 : 1;
+    if (!out || bytes < n * comps * 4) return fail(c, SPHE_EARG, "output buffer too small");
+    float* f = (float*)out;
+    switch (which) {
+    case SPHE_BUF_POS:
+        for (size_t i = 0; i < 3 * n; ++i) f[i] = (float)s->pos[i];
+        break;
+    case SPHE_BUF_VEL:
+        for (size_t i = 0; i < 3 * n; ++i) f[i] = (float)s->vel[i];
+        break;
+    case SPHE_BUF_MASS:
+        for (size_t i = 0; i < n; ++i) f[i] = s->prm.mass;
+        break;
+    case SPHE_BUF_SED: {
+        const double unit = std::ldexp(1.0, -s->fx_exp);
+        for (size_t i = 0; i < n; ++i) f[i] = (float)((double)s->sed[i] * unit);
+        break;
+    }
+    case SPHE_BUF_ID:
+        std::memcpy(out, s->ids.data(), n * 4);
+        break;
+    case SPHE_BUF_DENSITY:
+    case SPHE_BUF_PRESS: {
+        std::vector<double> rho;
+        r = read_rho(c, *s, rho);
+        if (r) return r;
+        for (size_t i = 0; i < n; ++i)
+            f[i] = which == SPHE_BUF_DENSITY ? (float)rho[i]
+                                             : s->prm.stiffness * ((float)rho[i] - s->prm.rest_density);
+        break;
+    }
+    default:
+        return fail(c, SPHE_EARG, "unknown buffer id %d", which);
+    }
+    return SPHE_OK;
+}
+
+int sphe_read_state_f64(sphe_ctx* c, int scene, double* pos3, double* vel3)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    int r = sync_to_host(c);
+    if (r) return r;
+    if (pos3) std::memcpy(pos3, s->pos.data(), s->pos.size() * 8);
+    if (vel3) std::memcpy(vel3, s->vel.data(), s->vel.size() * 8);
+    return SPHE_OK;
+}
+
+int sphe_read_sediment_fx(sphe_ctx* c, int scene, int64_t* sed)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    int r = sync_to_host(c);
+    if (r) return r;
+    if (!sed) return fail(c, SPHE_EARG, "null output");
+    std::memcpy(sed, s->sed.data(), s->sed.size() * 8);
+    return SPHE_OK;
+}
+
+int sphe_get_particle(sphe_ctx* c, int scene, int id, sphe_particle* out)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (!out) return fail(c, SPHE_EARG, "null output");
+    std::memset(out, 0, sizeof(*out));
+    int r = sync_to_host(c);
+    if (r) return r;
+    const auto it = std::lower_bound(s->ids.begin(), s->ids.end(), id);
+    if (it == s->ids.end() || *it != id) return fail(c, SPHE_ERANGE, "no particle with id %d", id);
+    const size_t hi = (size_t)(it - s->ids.begin());
+    out->Id = id;
+    out->NeighbId = -1;
+    for (int a = 0; a < 3; ++a) {
+        out->Position[a] = (float)s->pos[3 * hi + a];
+        out->Velocity[a] = (float)s->vel[3 * hi + a];
+    }
+    if (!c->stepped) return SPHE_OK;
+    // slot of the id in the last step's sorted buffers
+    const size_t n = s->ids.size();
+    std::vector<int32_t> sid(n);
+    HIPCHK(c, hipMemcpy(sid.data(), c->D.s_id + s->p_begin, n * 4, hipMemcpyDeviceToHost));
+    size_t slot = n;
+    for (size_t i = 0; i < n; ++i)
+        if (sid[i] == id) { slot = i; break; }
+    if (slot == n) return fail(c, SPHE_ERANGE, "id %d not in the last step", id);
+    sphe_launch_inspect(c->D, s->p_begin + (int64_t)slot, c->d_inspect, c->stream);
+    HIPCHK(c, hipGetLastError());
+    double v[22];
+    HIPCHK(c, hipMemcpyAsync(v, c->d_inspect, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out->Density = (float)v[0];
+    out->Pressure = (float)v[1];
+    for (int a = 0; a < 3; ++a) {
+        out->Acceleration[a] = (float)v[2 + a];
+        out->PressureForce[a] = (float)v[5 + a];
+        out->ViscosityForce[a] = (float)v[8 + a];
+        out->GravityForce[a] = (float)v[11 + a];
+        out->SurfaceForce[a] = (float)v[14 + a];
+        out->SurfaceNormal[a] = (float)v[17 + a];
+    }
+    out->NeighbId = (int32_t)v[20];
+    return SPHE_OK;
+}
+
+int sphe_terrain_load_u8(sphe_ctx* c, int scene, const uint8_t* img, int w, int h, float scale)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (!img || w <= 0 || h <= 0) return fail(c, SPHE_EARG, "bad image");
+    int r = begin_host_edit(c);
+    if (r) return r;
+    // Grid::GetHeightfieldAt(x, y) = map[dimY * x + y] (grid.h:104-107):
+    // terrain cell (x, z) reads map[h * x + z].
+    s->tnx = w;
+    s->tnz = h;
+    if (!(s->t_cell > 0.0f)) s->t_cell = 1.0f;
+    std::vector<float> hts((size_t)w * h);
+    for (int z = 0; z < h; ++z)
+        for (int x = 0; x < w; ++x) hts[(size_t)z * w + x] = s->t_org[1] + scale * (float)img[(size_t)h * x + z];
+    return set_terrain_heights(c, *s, hts.data());
+}
+
+int sphe_terrain_dims(const sphe_ctx* c, int scene, int* nx, int* nz)
+{
+    const Scene* s = scene_of(c, scene);
+    if (!s) return SPHE_EARG;
+    if (nx) *nx = s->tnx;
+    if (nz) *nz = s->tnz;
+    return SPHE_OK;
+}
+
+int sphe_fixed_exponent(const sphe_ctx* c, int scene, int* e)
+{
+    const Scene* s = scene_of(c, scene);
+    if (!s || !e) return SPHE_EARG;
+    *e = s->fx_exp;
+    return SPHE_OK;
+}
+
+int sphe_terrain_volume_fx(sphe_ctx* c, int scene, int64_t* out, size_t count)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    int r = sync_to_host(c);
+    if (r) return r;
+    if (!out || count < s->vol.size()) return fail(c, SPHE_EARG, "output too small");
+    std::memcpy(out, s->vol.data(), s->vol.size() * 8);
+    return SPHE_OK;
+}
+
+int sphe_terrain_heights(sphe_ctx* c, int scene, float* out, size_t count)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    const size_t nt = (size_t)s->tnx * s->tnz;
+    if (!out || count < nt) return fail(c, SPHE_EARG, "output too small");
+    if (nt == 0) return SPHE_OK;
+    int r = ensure_device(c);
+    if (r) return r;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, c->D.height + s->t_off, nt * 4, hipMemcpyDeviceToHost));
+    return SPHE_OK;
+}
+
+int sphe_terrain_mesh(sphe_ctx* c, int scene, float* verts, size_t* nverts, uint32_t* indices, size_t* nindices)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    const int nx = s->tnx, nz = s->tnz;
+    const size_t nv = (size_t)nx * nz * 6;
+    // genIndices (grid.h:118-136): two triangles per quad, the second walked
+    // from the far corner
+    size_t ni = 0;
+    for (int z = 0, j = nz - 1; z < nz && j >= 0; ++z, --j)
+        for (int x = 0, i = nx - 1; x < nx && i >= 0; ++x, --i) {
+            if (x + 1 < nx && z + 1 < nz) ni += 3;
+            if (j - 1 >= 0 && i - 1 >= 0) ni += 3;
+        }
+    if (nverts && !verts) *nverts = nv;
+    if (nindices && !indices) *nindices = ni;
+    if (!verts && !indices) return SPHE_OK;
+    if ((verts && (!nverts || *nverts < nv)) || (indices && (!nindices || *nindices < ni)))
+        return fail(c, SPHE_EARG, "mesh output too small");
+    std::vector<float> H(nv / 6 + 1);
+    if (nx > 0 && nz > 0) {
+        int r = sphe_terrain_heights(c, scene, H.data(), H.size());
+        if (r) return r;
+    }
+    if (verts) {
+        // UpdateGrid vertex stream (grid.h:138-176): position and the
+        // normalised sum of the four neighbour cross products
+        auto at = [&](int x, int z) { return H[(size_t)z * nx + x]; };
+        auto cross = [](const float* a, const float* b, float* o) {
+            o[0] = a[1] * b[2] - a[2] * b[1];
+            o[1] = a[2] * b[0] - a[0] * b[2];
+            o[2] = a[0] * b[1] - a[1] * b[0];
+        };
+        const float tc = s->t_cell;
+        size_t w = 0;
+        for (int z = 0; z < nz; ++z)
+            for (int x = 0; x < nx; ++x) {
+                const float y = at(x, z);
+                const float px = s->t_org[0] + ((float)x + 0.5f) * tc;
+                const float pz = s->t_org[2] + ((float)z + 0.5f) * tc;
+                float u[3] = {0, 0, 0}, d[3] = {0, 0, 0}, r3[3] = {0, 0, 0}, l[3] = {0, 0, 0};
+                if (x - 1 >= 0) { l[0] = tc; l[1] = y - at(x - 1, z); }
+                if (x + 1 < nx) { r3[0] = tc; r3[1] = at(x + 1, z) - y; }
+                if (z - 1 >= 0) { u[1] = y - at(x, z - 1); u[2] = tc; }
+                if (z + 1 < nz) { d[1] = at(x, z + 1) - y; d[2] = tc; }
+                float a1[3], a2[3], a3[3], a4[3];
+                cross(u, l, a1);
+                cross(u, r3, a2);
+                cross(d, l, a3);
+                cross(d, r3, a4);
+                float n3[3] = {a1[0] + a2[0] + a3[0] + a4[0], a1[1] + a2[1] + a3[1] + a4[1],
+                               a1[2] + a2[2] + a3[2] + a4[2]};
+                const float len = std::sqrt((n3[0] * n3[0] + n3[1] * n3[1]) + n3[2] * n3[2]);
+                const float inv = len > 0.0f ? 1.0f / len : 0.0f;
+                verts[w++] = px;
+                verts[w++] = y;
+                verts[w++] = pz;
+                verts[w++] = n3[0] * inv;
+                verts[w++] = n3[1] * inv;
+                verts[w++] = n3[2] * inv;
+            }
+        *nverts = nv;
+    }
+    if (indices) {
+        size_t w = 0;
+        for (int z = 0, j = nz - 1; z < nz && j >= 0; ++z, --j)
+            for (int x = 0, i = nx - 1; x < nx && i >= 0; ++x, --i) {
+                if (x + 1 < nx && z + 1 < nz) {
+                    indices[w++] = (uint32_t)(z * nx + x);
+                    indices[w++] = (uint32_t)(z * nx + (x + 1));
+                    indices[w++] = (uint32_t)((z + 1) * nx + x);
+                }
+                if (j - 1 >= 0 && i - 1 >= 0) {
+                    indices[w++] = (uint32_t)(j * nx + i);
+                    indices[w++] = (uint32_t)(j * nx + (i - 1));
+                    indices[w++] = (uint32_t)((j - 1) * nx + i);
+                }
+            }
+        *nindices = ni;
+    }
+    return SPHE_OK;
+}
+
+int sphe_step(sphe_ctx* c, int nsteps)
+{
+    if (!c) return SPHE_EARG;
+    if (nsteps < 0) return fail(c, SPHE_EARG, "negative step count");
+    if (nsteps == 0) return SPHE_OK;
+    return do_steps(c, nsteps);
+}
+
+int sphe_sync(sphe_ctx* c)
+{
+    if (!c) return SPHE_EARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return collect_profile(c);
+}
+
+int sphe_use_graph(sphe_ctx* c, int enable)
+{
+    if (!c) return SPHE_EARG;
+    c->use_graph = enable != 0;
+    return SPHE_OK;
+}
+
+int sphe_mass(sphe_ctx* c, int scene, double* terrain, double* sediment, uint64_t* units)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    int r = prepare(c);
+    if (r) return r;
+    sphe_launch_mass(c->D, scene, c->d_mass_part, c->d_mass_out, c->stream);
+    HIPCHK(c, hipGetLastError());
+    double v[3];
+    HIPCHK(c, hipMemcpyAsync(v, c->d_mass_out, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (terrain) *terrain = v[0];
+    if (sediment) *sediment = v[1];
+    if (units) std::memcpy(units, &v[2], 8);
+    (void)s;
+    return SPHE_OK;
+}
+
+int sphe_status(sphe_ctx* c, uint32_t* nonfinite)
+{
+    if (!c || !nonfinite) return SPHE_EARG;
+    *nonfinite = 0;
+    if (!c->layout_valid) return SPHE_OK;
+    HIPCHK(c, hipMemcpyAsync(nonfinite, c->D.status, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SPHE_OK;
+}
+
+int sphe_profile(sphe_ctx* c, int enable)
+{
+    if (!c) return SPHE_EARG;
+    int r = collect_profile(c);
+    if (r) return r;
+    c->profiling = enable != 0;
+    c->prof.clear();
+    return SPHE_OK;
+}
+
+int sphe_profile_read(sphe_ctx* c, const char* kernel, double* avg_ms, int* launches)
+{
+    if (!c || !kernel) return SPHE_EARG;
+    int r = collect_profile(c);
+    if (r) return r;
+    auto it = c->prof.find(kernel);
+    if (it == c->prof.end()) {
+        if (avg_ms) *avg_ms = 0.0;
+        if (launches) *launches = 0;
+        return SPHE_OK;
+    }
+    if (avg_ms) *avg_ms = it->second.second ? it->second.first / it->second.second : 0.0;
+    if (launches) *launches = it->second.second;
+    return SPHE_OK;
+}
+
+int sphe_profile_names(sphe_ctx* c, char* names, size_t cap)
+{
+    if (!c || !names || cap == 0) return SPHE_EARG;
+    int r = collect_profile(c);
+    if (r) return r;
+    std::string s;
+    for (auto& kv : c->prof) {
+        if (!s.empty()) s += ",";
+        s += kv.first;
+    }
+    std::snprintf(names, cap, "%s", s.c_str());
+    return SPHE_OK;
+}
+
+int sphe_kernel_bytes(sphe_ctx* c, const char* kernel, double* bytes)
+{
+    if (!c || !kernel || !bytes) return SPHE_EARG;
+    const double E = (double)real_size(c->precision);
+    double N = 0, cells = 0, tcells = 0, terr_n = 0, ero_n = 0;
+    for (const Scene& s : c->scenes) {
+        const double n = (double)s.ids.size();
+        N += n;
+        int dims[3];
+        grid_dims(s.prm, dims);
+        cells += (double)dims[0] * dims[1] * dims[2] + 1.0;
+        tcells += (double)s.tnx * s.tnz;
+        if (s.tnx > 0) terr_n += n;
+        if (s.tnx > 0 && s.prm.erosion) ero_n += n;
+    }
+    const std::string k = kernel;
+    // compulsory inputs and outputs of each pass (intermediates kept in
+    // registers or on chip are not counted)
+    if (k == "density") *bytes = N * (3 * E + E) + 4 * cells;           // pos -> rho; start table
+    else if (k == "force")
+        *bytes = N * (3 * E + 3 * E + E) + N * (3 * E + 3 * E) + N * 8  // pos, vel, rho -> pos, vel; key+rank
+                 + terr_n * 4 + ero_n * (8 + 8);                        // heights; sediment in/out
+    else if (k == "reorder") *bytes = N * (12 + 2 * (6 * E + 12));       // tmp + gather + write
+    else if (k == "scatter") *bytes = N * (12 + 12 + 4);
+    else if (k == "scan") *bytes = 12 * cells;
+    else if (k == "heights") *bytes = 12 * tcells;
+    else if (k == "hash") *bytes = N * (3 * E + 8);
+    else return fail(c, SPHE_EARG, "unknown kernel %s", kernel);
+    return SPHE_OK;
+}
+
+int sphe_set_tuning(sphe_ctx* c, int block, int remap)
+{
+    if (!c) return SPHE_EARG;
+    if (block != 64 && block != 128 && block != 256 && block != 512)
+        return fail(c, SPHE_EARG, "block must be 64, 128, 256 or 512");
+    c->block = block;
+    c->remap = remap ? 1 : 0;
+    return SPHE_OK;
+}
+
+int sphe_get_tuning(const sphe_ctx* c, int* block, int* remap)
+{
+    if (!c) return SPHE_EARG;
+    if (block) *block = c->block;
+    if (remap) *remap = c->remap;
+    return SPHE_OK;
+}
+
+int sphe_autotune(sphe_ctx* c, int steps_per_trial)
+{
+    if (!c) return SPHE_EARG;
+    if (steps_per_trial < 1) steps_per_trial = 1;
+    static std::mutex mu;
+    static std::map<std::string, std::pair<int, int>> cache;
+    int r = prepare(c);
+    if (r) return r;
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    std::string key = std::string(prop.gcnArchName) + "|" + plan_signature(c).substr(0, 64);
+    for (const Scene& s : c->scenes) key += "|" + std::to_string(s.ids.size()) + ":" + std::to_string(s.prm.h);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            c->block = it->second.first;
+            c->remap = it->second.second;
+            return SPHE_OK;
+        }
+    }
+    r = sync_to_host(c);
+    if (r) return r;
+    const bool prof = c->profiling;
+    c->profiling = false;
+    int best_b = c->block, best_r = c->remap;
+    double best = 1e30;
+    hipEvent_t e0 = get_event(c), e1 = get_event(c);
+    for (int b : {64, 128, 256, 512})
+        for (int rm : {0, 1}) {
+            c->block = b;
+            c->remap = rm;
+            c->device_valid = false;  // every trial starts from the same state
+            r = do_steps(c, 1);
+            if (r) return r;
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+            r = do_steps(c, steps_per_trial);
+            if (r) return r;
+            HIPCHK(c, hipEventRecord(e1, c->stream));
+            HIPCHK(c, hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; best_b = b; best_r = rm; }
+        }
+    c->event_pool.push_back(e0);
+    c->event_pool.push_back(e1);
+    c->block = best_b;
+    c->remap = best_r;
+    c->profiling = prof;
+    c->host_valid = true;     // host staging still holds the pre-tuning state
+    c->device_valid = false;  // restore it on the next use
+    c->stepped = false;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        cache[key] = {best_b, best_r};
+    }
+    return SPHE_OK;
+}
+
+int sphe_debug_cell_keys(sphe_ctx* c, int scene, int32_t* keys)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (!keys) return fail(c, SPHE_EARG, "null output");
+    int r = prepare(c);
+    if (r) return r;
+    const size_t n = s->ids.size();
+    if (n == 0) return SPHE_OK;
+    int32_t* dk = nullptr;
+    HIPCHK(c, hipMalloc(&dk, (size_t)s->p_cap * 4));
+    sphe_launch_keys(c->D, scene, dk, c->stream);
+    std::vector<int32_t> k(n), id(n);
+    hipError_t e1 = hipMemcpyAsync(k.data(), dk, n * 4, hipMemcpyDeviceToHost, c->stream);
+    hipError_t e2 = hipMemcpyAsync(id.data(), c->D.u_id + s->p_begin, n * 4, hipMemcpyDeviceToHost, c->stream);
+    hipError_t e3 = hipStreamSynchronize(c->stream);
+    (void)hipFree(dk);
+    if (e1 || e2 || e3) return fail(c, SPHE_EHIP, "reading cell keys failed");
+    std::vector<size_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) { return id[a] < id[b]; });
+    for (size_t t = 0; t < n; ++t) keys[t] = k[perm[t]];
+    return SPHE_OK;
+}
+
+int sphe_debug_consts(const sphe_ctx* c, int scene, double* out, int cap)
+{
+    const Scene* s = scene_of(c, scene);
+    if (!s || !out || cap < 20) return SPHE_EARG;
+    SceneDev d;
+    std::memset(&d, 0, sizeof(d));
+    derive(*s, d);
+    const double v[20] = {d.f.h, d.f.h2, d.f.thr, d.f.t3h2, d.f.mc6, d.f.pf_scale, d.f.vf_scale,
+                          d.f.ng, d.f.h * 1.00390625f, d.f.inv_cell, d.f.org[0], d.f.org[1],
+                          d.f.org[2], (double)d.dims[0], (double)d.dims[1], (double)d.dims[2],
+                          d.hq, d.f.kcv, d.f.up, d.f.dn};
+    std::memcpy(out, v, sizeof(v));
+    return SPHE_OK;
+}
+
+int sphe_debug_neighbours(sphe_ctx* c, int scene, int32_t* counts, const int64_t* offsets, int32_t* idx)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (!counts) return fail(c, SPHE_EARG, "null counts");
+    int r = sort_current(c);
+    if (r) return r;
+    const size_t n = s->ids.size();
+    if (n == 0) return SPHE_OK;
+    std::vector<int32_t> sid(n), cnt(n);
+    HIPCHK(c, hipMemcpy(sid.data(), c->D.s_id + s->p_begin, n * 4, hipMemcpyDeviceToHost));
+    int32_t* dcnt = nullptr;
+    HIPCHK(c, hipMalloc(&dcnt, n * 4));
+    sphe_launch_neighbours(c->D, scene, nullptr, dcnt, nullptr, c->stream);
+    hipError_t e = hipMemcpyAsync(cnt.data(), dcnt, n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(dcnt);
+        return fail(c, SPHE_EHIP, "neighbour counts: %s", hipGetErrorString(e));
+    }
+    std::vector<size_t> perm(n);  // perm[t] = sorted slot of the t-th id
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) { return sid[a] < sid[b]; });
+    for (size_t t = 0; t < n; ++t) counts[t] = cnt[perm[t]];
+    if (!idx || !offsets) {
+        (void)hipFree(dcnt);
+        return SPHE_OK;
+    }
+    std::vector<int64_t> soff(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) soff[i + 1] = soff[i] + cnt[i];
+    int64_t* doff = nullptr;
+    int32_t* didx = nullptr;
+    e = hipMalloc(&doff, (n + 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&didx, (size_t)std::max<int64_t>(1, soff[n]) * 4);
+    if (e == hipSuccess) e = hipMemcpy(doff, soff.data(), (n + 1) * 8, hipMemcpyHostToDevice);
+    std::vector<int32_t> hidx((size_t)soff[n]);
+    if (e == hipSuccess) {
+        sphe_launch_neighbours(c->D, scene, doff, dcnt, didx, c->stream);
+        e = hipMemcpyAsync(hidx.data(), didx, (size_t)soff[n] * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    (void)hipFree(dcnt);
+    (void)hipFree(doff);
+    (void)hipFree(didx);
+    if (e != hipSuccess) return fail(c, SPHE_EHIP, "neighbour lists: %s", hipGetErrorString(e));
+    for (size_t t = 0; t < n; ++t) {
+        const size_t sl = perm[t];
+        int32_t* dst = idx + offsets[t];
+        std::memcpy(dst, hidx.data() + soff[sl], (size_t)cnt[sl] * 4);
+        std::sort(dst, dst + cnt[sl]);
+    }
+    return SPHE_OK;
+}
+
+int sphe_debug_density(sphe_ctx* c, int scene, double* rho)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (!rho) return fail(c, SPHE_EARG, "null output");
+    int r = sort_current(c);
+    if (r) return r;
+    c->stepped = true;  // sorted buffers and rho now describe the current state
+    std::vector<double> v;
+    r = read_rho(c, *s, v);
+    c->stepped = false;
+    if (r) return r;
+    std::memcpy(rho, v.data(), v.size() * 8);
+    return SPHE_OK;
+}
+
+}  // extern "C"

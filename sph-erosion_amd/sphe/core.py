@@ -1,0 +1,286 @@
+"""Python mirror of the C ABI (include/sphe.h) for tests and benchmarks.
+
+Context wraps one sphe_ctx: a batch of scenes advanced together on the GPU.
+All compute runs in libsphe.so; this module only moves host arrays across the
+boundary.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (BUF_DENSITY, BUF_ID, BUF_MASS, BUF_POS, BUF_PRESS, BUF_SED, BUF_VEL, FP32, FP64,
+                   Params, Particle, TerrainDesc, check, lib)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def default_params(**kw):
+    """Reference defaults (Erosion/fluid_system.h:460-478), overridden by kw."""
+    p = Params()
+    lib().sphe_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k in ("gravity", "box_center"):
+            arr = getattr(p, k)
+            for i, x in enumerate(v):
+                arr[i] = float(x)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def params_dict(p):
+    return {k: (list(getattr(p, k)) if k in ("gravity", "box_center") else getattr(p, k))
+            for k, _ in Params._fields_}
+
+
+class Context:
+    def __init__(self, precision=FP32):
+        self.precision = precision
+        self._p = C.c_void_p()
+        check(lib().sphe_create(precision, C.byref(self._p)), None, "sphe_create")
+        self._terrains = []
+
+    # ---- lifetime
+    def close(self):
+        if self._p:
+            lib().sphe_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        return check(rc, self._p, what)
+
+    # ---- scenes
+    def add_scene(self, params, terrain=None):
+        td = None
+        if terrain is not None:
+            hts = np.ascontiguousarray(terrain["heights"], dtype=np.float32)
+            self._terrains.append(hts)
+            td = TerrainDesc()
+            td.nx, td.nz = int(terrain["nx"]), int(terrain["nz"])
+            for i in range(3):
+                td.origin[i] = float(terrain["origin"][i])
+            td.cell = float(terrain["cell"])
+            td.heights = hts.ctypes.data
+        sid = C.c_int()
+        self._chk(lib().sphe_add_scene(self._p, C.byref(params), C.byref(td) if td else None,
+                                       C.byref(sid)), "add_scene")
+        return sid.value
+
+    def add_synthetic(self, sc):
+        """Adds a scenes.Scene (params, terrain and particle state)."""
+        sid = self.add_scene(sc.params, sc.terrain)
+        self.set_particles(sid, sc.pos, sc.vel, sc.ids)
+        return sid
+
+    def set_params(self, scene, params):
+        self._chk(lib().sphe_set_params(self._p, scene, C.byref(params)), "set_params")
+
+    def get_params(self, scene):
+        p = Params()
+        self._chk(lib().sphe_get_params(self._p, scene, C.byref(p)), "get_params")
+        return p
+
+    def set_particles(self, scene, pos, vel=None, ids=None, sed=None):
+        n = len(pos)
+        ids = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+        sed = None if sed is None else np.ascontiguousarray(sed, dtype=np.int64)
+        if self.precision == FP64:
+            pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(n, 3)
+            vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(n, 3)
+            rc = lib().sphe_set_particles_f64(self._p, scene, n, _ptr(pos), _ptr(vel), _ptr(ids), _ptr(sed))
+        else:
+            pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(n, 3)
+            vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(n, 3)
+            rc = lib().sphe_set_particles(self._p, scene, n, _ptr(pos), _ptr(vel), _ptr(ids), _ptr(sed))
+        self._chk(rc, "set_particles")
+
+    def num_particles(self, scene):
+        n = C.c_int()
+        self._chk(lib().sphe_num_particles(self._p, scene, C.byref(n)), "num_particles")
+        return n.value
+
+    # ---- stepping
+    def step(self, n=1):
+        self._chk(lib().sphe_step(self._p, n), "step")
+
+    def sync(self):
+        self._chk(lib().sphe_sync(self._p), "sync")
+
+    def use_graph(self, on):
+        self._chk(lib().sphe_use_graph(self._p, int(on)), "use_graph")
+
+    # ---- reads (ascending id order)
+    def read(self, scene, which):
+        n = self.num_particles(scene)
+        if which in (BUF_POS, BUF_VEL):
+            out = np.zeros((n, 3), dtype=np.float32)
+        elif which == BUF_ID:
+            out = np.zeros(n, dtype=np.int32)
+        else:
+            out = np.zeros(n, dtype=np.float32)
+        self._chk(lib().sphe_read_buffer(self._p, scene, which, _ptr(out), out.nbytes), "read_buffer")
+        return out
+
+    def positions(self, scene):
+        return self.read(scene, BUF_POS)
+
+    def velocities(self, scene):
+        return self.read(scene, BUF_VEL)
+
+    def ids(self, scene):
+        return self.read(scene, BUF_ID)
+
+    def density(self, scene):
+        return self.read(scene, BUF_DENSITY)
+
+    def pressure(self, scene):
+        return self.read(scene, BUF_PRESS)
+
+    def sediment(self, scene):
+        return self.read(scene, BUF_SED)
+
+    def mass_buffer(self, scene):
+        return self.read(scene, BUF_MASS)
+
+    def state_f64(self, scene):
+        n = self.num_particles(scene)
+        pos = np.zeros((n, 3))
+        vel = np.zeros((n, 3))
+        self._chk(lib().sphe_read_state_f64(self._p, scene, _ptr(pos), _ptr(vel)), "read_state_f64")
+        return pos, vel
+
+    def sediment_fx(self, scene):
+        out = np.zeros(self.num_particles(scene), dtype=np.int64)
+        self._chk(lib().sphe_read_sediment_fx(self._p, scene, _ptr(out)), "read_sediment_fx")
+        return out
+
+    def particle(self, scene, pid):
+        p = Particle()
+        self._chk(lib().sphe_get_particle(self._p, scene, pid, C.byref(p)), "get_particle")
+        return p
+
+    # ---- terrain
+    def terrain_dims(self, scene):
+        nx, nz = C.c_int(), C.c_int()
+        self._chk(lib().sphe_terrain_dims(self._p, scene, C.byref(nx), C.byref(nz)), "terrain_dims")
+        return nx.value, nz.value
+
+    def terrain_heights(self, scene):
+        nx, nz = self.terrain_dims(scene)
+        out = np.zeros((nz, nx), dtype=np.float32)
+        self._chk(lib().sphe_terrain_heights(self._p, scene, _ptr(out), out.size), "terrain_heights")
+        return out
+
+    def terrain_volume_fx(self, scene):
+        nx, nz = self.terrain_dims(scene)
+        out = np.zeros((nz, nx), dtype=np.int64)
+        self._chk(lib().sphe_terrain_volume_fx(self._p, scene, _ptr(out), out.size), "terrain_volume_fx")
+        return out
+
+    def fixed_exponent(self, scene):
+        e = C.c_int()
+        self._chk(lib().sphe_fixed_exponent(self._p, scene, C.byref(e)), "fixed_exponent")
+        return e.value
+
+    def terrain_load_u8(self, scene, img, scale):
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        w, h = img.shape[0], img.shape[1]
+        self._chk(lib().sphe_terrain_load_u8(self._p, scene, _ptr(img), w, h, float(scale)), "terrain_load_u8")
+
+    def terrain_mesh(self, scene):
+        nv, ni = C.c_size_t(), C.c_size_t()
+        self._chk(lib().sphe_terrain_mesh(self._p, scene, None, C.byref(nv), None, C.byref(ni)), "mesh sizes")
+        verts = np.zeros(nv.value, dtype=np.float32)
+        idx = np.zeros(ni.value, dtype=np.uint32)
+        self._chk(lib().sphe_terrain_mesh(self._p, scene, _ptr(verts), C.byref(nv), _ptr(idx), C.byref(ni)),
+                  "terrain_mesh")
+        return verts.reshape(-1, 6), idx
+
+    # ---- bookkeeping
+    def mass(self, scene):
+        t, s, u = C.c_double(), C.c_double(), C.c_uint64()
+        self._chk(lib().sphe_mass(self._p, scene, C.byref(t), C.byref(s), C.byref(u)), "mass")
+        return t.value, s.value, u.value
+
+    def status(self):
+        v = C.c_uint32()
+        self._chk(lib().sphe_status(self._p, C.byref(v)), "status")
+        return v.value
+
+    # ---- measurement and tuning
+    def profile(self, on):
+        self._chk(lib().sphe_profile(self._p, int(on)), "profile")
+
+    def profile_read(self, kernel):
+        ms, n = C.c_double(), C.c_int()
+        self._chk(lib().sphe_profile_read(self._p, kernel.encode(), C.byref(ms), C.byref(n)), "profile_read")
+        return ms.value, n.value
+
+    def profile_names(self):
+        buf = C.create_string_buffer(1024)
+        self._chk(lib().sphe_profile_names(self._p, buf, 1024), "profile_names")
+        return [x for x in buf.value.decode().split(",") if x]
+
+    def kernel_bytes(self, kernel):
+        b = C.c_double()
+        self._chk(lib().sphe_kernel_bytes(self._p, kernel.encode(), C.byref(b)), "kernel_bytes")
+        return b.value
+
+    def set_tuning(self, block, remap):
+        self._chk(lib().sphe_set_tuning(self._p, block, int(remap)), "set_tuning")
+
+    def get_tuning(self):
+        b, r = C.c_int(), C.c_int()
+        self._chk(lib().sphe_get_tuning(self._p, C.byref(b), C.byref(r)), "get_tuning")
+        return b.value, r.value
+
+    def autotune(self, steps_per_trial=5):
+        self._chk(lib().sphe_autotune(self._p, steps_per_trial), "autotune")
+        return self.get_tuning()
+
+    # ---- verification hooks
+    def debug_cell_keys(self, scene):
+        out = np.zeros(self.num_particles(scene), dtype=np.int32)
+        self._chk(lib().sphe_debug_cell_keys(self._p, scene, _ptr(out)), "debug_cell_keys")
+        return out
+
+    def debug_consts(self, scene):
+        out = np.zeros(20)
+        self._chk(lib().sphe_debug_consts(self._p, scene, _ptr(out), 20), "debug_consts")
+        keys = ["h", "h2", "thr", "t3h2", "mc6", "pf_scale", "vf_scale", "ng", "cell", "inv_cell",
+                "org_x", "org_y", "org_z", "dim_x", "dim_y", "dim_z", "hq", "kcv", "up", "dn"]
+        return dict(zip(keys, out))
+
+    def debug_neighbours(self, scene, csr=True):
+        n = self.num_particles(scene)
+        counts = np.zeros(n, dtype=np.int32)
+        self._chk(lib().sphe_debug_neighbours(self._p, scene, _ptr(counts), None, None), "debug_neighbours")
+        if not csr:
+            return counts, None, None
+        off = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(counts, out=off[1:])
+        idx = np.zeros(max(int(off[-1]), 1), dtype=np.int32)
+        self._chk(lib().sphe_debug_neighbours(self._p, scene, _ptr(counts), _ptr(off), _ptr(idx)),
+                  "debug_neighbours")
+        return counts, off, idx[:off[-1]]
+
+    def debug_density(self, scene):
+        out = np.zeros(self.num_particles(scene))
+        self._chk(lib().sphe_debug_density(self._p, scene, _ptr(out)), "debug_density")
+        return out
