@@ -168,6 +168,11 @@ int sphe_get_particle(sphe_ctx* ctx, int scene, int id, sphe_particle* out);
 
 /* ---- terrain ----------------------------------------------------------- */
 
+/* Attaches (or replaces, or with nx == 0 removes) a scene's heightfield;
+ * heights may be NULL for a flat terrain at origin[1].  The reference's
+ * front end holds its heightfield in Grid (grid.h:26-51) and never couples it
+ * to the fluid; this is the coupling. */
+int sphe_set_terrain(sphe_ctx* ctx, int scene, const sphe_terrain_desc* terrain);
 /* Grid::LoadHeightfield (grid.h:98-102): uint8 heights of a w x h image,
  * map[w_index * h + h_index] as Grid::GetHeightfieldAt reads it
  * (grid.h:104-107), scaled by `scale` and offset by the terrain origin y. */

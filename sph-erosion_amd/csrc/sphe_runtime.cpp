@@ -1202,6 +1202,24 @@ int sphe_get_particle(sphe_ctx* c, int scene, int id, sphe_particle* out)
     return SPHE_OK;
 }
 
+int sphe_set_terrain(sphe_ctx* c, int scene, const sphe_terrain_desc* t)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (t && t->nx > 0 && t->nz > 0 && !(t->cell > 0.0f)) return fail(c, SPHE_EARG, "terrain cell must be > 0");
+    int r = begin_host_edit(c);
+    if (r) return r;
+    if (!t || t->nx <= 0 || t->nz <= 0) {
+        s->tnx = s->tnz = 0;
+        s->vol.clear();
+        return SPHE_OK;
+    }
+    s->tnx = t->nx;
+    s->tnz = t->nz;
+    for (int a = 0; a < 3; ++a) s->t_org[a] = t->origin[a];
+    s->t_cell = t->cell;
+    return set_terrain_heights(c, *s, t->heights);
+}
+
 int sphe_terrain_load_u8(sphe_ctx* c, int scene, const uint8_t* img, int w, int h, float scale)
 {
     SCENE_OR_FAIL(s, c, scene);

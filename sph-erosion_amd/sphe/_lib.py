@@ -74,6 +74,7 @@ SIGNATURES = {
     "sphe_read_state_f64": (I, [P, I, P, P]),
     "sphe_read_sediment_fx": (I, [P, I, P]),
     "sphe_get_particle": (I, [P, I, I, C.POINTER(Particle)]),
+    "sphe_set_terrain": (I, [P, I, C.POINTER(TerrainDesc)]),
     "sphe_terrain_load_u8": (I, [P, I, P, I, I, C.c_float]),
     "sphe_terrain_heights": (I, [P, I, P, C.c_size_t]),
     "sphe_terrain_volume_fx": (I, [P, I, P, C.c_size_t]),

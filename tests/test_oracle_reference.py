@@ -1,0 +1,199 @@
+"""The oracle's ref32 restatement against the reference step.
+
+Pins: the golden fixtures written by the reference's own FluidSystemSPH
+(tests/golden/make_golden.py), a live comparison with oracle/_ref when it was
+built here, and closed forms that a dropped term, a wrong sign or a wrong
+constant would break.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+PI_F = float(np.float32(3.141592))  # Erosion/sphere.h:8
+H = float(np.float32(0.0457))
+M = float(np.float32(0.02))
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+def recs(buf):
+    return np.frombuffer(buf.tobytes(), dtype=O.PARTICLE_DTYPE)
+
+
+def assert_records_equal(a, b, fields=O.PARTICLE_DTYPE.names):
+    for f in fields:
+        assert a[f].tobytes() == b[f].tobytes(), f
+
+
+def test_lattice_fixtures():
+    g = load("lattice.npz")
+    for name, (n, add) in {"l1000": (1000, 0), "l100": (100, 0), "l125_125": (125, 125)}.items():
+        L = O.lattice(n)
+        if add:
+            L = np.concatenate([L, O.lattice(add, first_id=len(L))])
+        for f in ("Id", "Position", "Velocity", "Acceleration"):
+            assert L[f].tobytes() == g[f"{name}_{f}"].tobytes(), (name, f)
+    # Initialize(100): i < cbrt(100) = 4.64 gives 5 per axis
+    assert len(O.lattice(100)) == 125
+
+
+@pytest.mark.parametrize("steps,key", [(1, "step1"), (10, "step10")])
+def test_f1_lattice_bit_exact(steps, key):
+    ref = recs(load("f1_lattice.npz")[key])
+    out = O.ref32_step(O.lattice(1000), O.default_params(dt=0.01), steps)
+    assert_records_equal(out, ref)
+
+
+@pytest.mark.parametrize("steps,key", [(1, "step1"), (3, "step3")])
+def test_f2_coincident_bit_exact(steps, key):
+    ref = recs(load("f2_coincident.npz")[key])
+    L = np.concatenate([O.lattice(125), O.lattice(125, first_id=125)])
+    out = O.ref32_step(L, O.default_params(dt=0.01), steps)
+    assert_records_equal(out, ref)
+
+
+def test_f3_jitter_bit_exact():
+    g = load("f3_jitter.npz")
+    P = np.zeros(len(g["in_pos"]), dtype=O.PARTICLE_DTYPE)
+    P["Id"] = np.arange(len(P))
+    P["NeighbId"] = -1
+    P["Position"] = g["in_pos"]
+    P["Velocity"] = g["in_vel"]
+    p = O.default_params(dt=1e-3)
+    p.len, p.p0 = 0.5, 1000.0
+    p.g[1] = -9.8
+    out = O.ref32_step(P, p, 1)
+    for f in ("Position", "Velocity", "Acceleration", "Density", "Pressure", "NeighbId", "PressureForce",
+              "SurfaceNormal"):
+        assert out[f].tobytes() == g[f].tobytes(), f
+
+
+def test_f4_box_bit_exact():
+    g = load("f4_box.npz")
+    P = np.zeros(len(g["in_pos"]), dtype=O.PARTICLE_DTYPE)
+    P["Id"] = np.arange(len(P))
+    P["NeighbId"] = -1
+    P["Position"] = g["in_pos"]
+    P["Velocity"] = g["in_vel"]
+    out = O.ref32_step(P, O.default_params(dt=0.01), 1)
+    for f in ("Position", "Velocity", "Acceleration", "Density", "NeighbId"):
+        assert out[f].tobytes() == g[f].tobytes(), f
+    # a particle outside the box with |v'| = 0 gets NaN velocity: the
+    # response divides by |v'| (fluid_system.h:345)
+    assert np.isnan(out["Velocity"][11]).all()
+    assert np.isfinite(out["Velocity"][:11]).all()
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (no reference checkout)")
+@pytest.mark.parametrize("seed", [1, 2])
+def test_live_reference_random_states(seed):
+    rng = np.random.default_rng(seed)
+    n = 700
+    P = np.zeros(n, dtype=O.PARTICLE_DTYPE)
+    P["Id"] = np.arange(n)
+    P["NeighbId"] = -1
+    P["Position"] = rng.uniform(-0.12, 0.12, size=(n, 3)).astype(np.float32)
+    P["Velocity"] = rng.normal(0, 0.5, size=(n, 3)).astype(np.float32)
+    # a few coincident pairs
+    P["Position"][1:20:2] = P["Position"][0:19:2]
+    p = O.default_params(dt=0.005)
+    a = O.ref32_step(P, p, 5)
+    b = O.refbin_step(P, p, 5)
+    assert_records_equal(a, b)
+
+
+def test_isolated_particle_closed_form():
+    """Density of an isolated particle is its self term m 315/(64 pi h^3);
+    it free-falls with a = g exactly."""
+    P = np.zeros(1, dtype=O.PARTICLE_DTYPE)
+    p = O.default_params(dt=0.01)
+    out = O.ref32_step(P, p, 1)
+    rho = M * 315.0 / (64.0 * PI_F * H ** 3)
+    assert abs(out["Density"][0] - rho) <= 2e-6 * rho
+    assert abs(rho - 328.2934) < 1e-3
+    np.testing.assert_array_equal(out["Acceleration"][0], np.float32([0.0, -9.82, 0.0]))
+    assert out["Velocity"][0][1] == np.float32(np.float32(-9.82) * np.float32(0.01))
+
+
+def test_lattice_interior_density_closed_form():
+    """Interior particle of the reference lattice (spacing 0.025): 26
+    neighbours at distances 1, sqrt 2, sqrt 3 spacings plus itself."""
+    s = 0.025
+    offs = [(i, j, k) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)]
+    c = 315.0 / (64.0 * PI_F * H ** 9)
+    rho = sum(M * c * (H * H - (i * i + j * j + k * k) * s * s) ** 3 for i, j, k in offs)
+    assert abs(rho - 1263.8149) < 1e-3
+    out = O.ref32_step(O.lattice(1000), O.default_params(dt=0.01), 1)
+    interior = [555, 444, 333]
+    for i in interior:
+        assert abs(out["Density"][i] - rho) <= 1e-5 * rho
+        assert abs(out["Pressure"][i] - 3.0 * (rho - 998.29)) <= 1e-3 * 3.0 * (rho - 998.29)
+
+
+def test_box_response_closed_form():
+    """x' = 0.25 beyond the +x wall at len 0.2 with v = (1,0,0), dt = 0.01:
+    d = 0.05, v' = v - n (1 + 0.5 d / (dt |v|)) (v.n) = (-2.5, 0, 0), x' = 0.2."""
+    P = np.zeros(1, dtype=O.PARTICLE_DTYPE)
+    P["Position"] = [[0.24, 0.0, 0.0]]
+    P["Velocity"] = [[1.0, 0.0, 0.0]]
+    p = O.default_params(dt=0.01)
+    p.g[1] = 0.0
+    out = O.ref32_step(P, p, 1)
+    assert out["Position"][0][0] == np.float32(0.2)
+    assert abs(out["Velocity"][0][0] - (-2.5)) < 1e-5
+    assert out["Velocity"][0][1] == 0.0 and out["Velocity"][0][2] == 0.0
+
+
+def test_two_particle_pressure_antisymmetry():
+    """Two particles at distance 0.02 along x: equal and opposite pressure
+    forces of magnitude (p_i/rho_i^2 + p_j/rho_j^2) m 45/(pi h^6) (h-r)^2 rho."""
+    P = np.zeros(2, dtype=O.PARTICLE_DTYPE)
+    P["Id"] = [0, 1]
+    P["Position"] = [[0.0, 0.0, 0.0], [0.02, 0.0, 0.0]]
+    p = O.default_params(dt=0.0)
+    out = O.ref32_step(P, p, 1)
+    r = float(np.float32(0.02))
+    c6 = 315.0 / (64.0 * PI_F * H ** 9)
+    rho = M * c6 * (H ** 6 + (H * H - r * r) ** 3)
+    pr = 3.0 * (rho - float(np.float32(998.29)))
+    mag = 2.0 * pr / rho ** 2 * M * 45.0 / (PI_F * H ** 6) * (H - r) ** 2 * rho
+    pf = out["PressureForce"]
+    assert abs(pf[0][0] + pf[1][0]) <= 1e-5 * abs(pf[0][0])
+    assert abs(abs(pf[0][0]) - abs(mag)) <= 1e-4 * abs(mag)
+    # pressure below rest density pulls the pair together: particle 0 is
+    # pulled toward +x
+    assert pr < 0 and pf[0][0] > 0
+
+
+def test_predicate_threshold_exhaustive_walk():
+    """T(h) is the largest float with sqrtf(T) <= h: checked by walking the
+    floats around it, and the predicate d2 <= T equals sqrtf(d2) <= h."""
+    for h in (0.0457, 0.0083984375, 0.00619, 0.1, 1.0, 2.15):
+        hf = np.float32(h)
+        T = np.float32(O.pred_threshold(hf))
+        assert np.sqrt(T) <= hf
+        assert np.sqrt(np.nextafter(T, np.float32(np.inf))) > hf
+        walk = T.view(np.uint32) + np.arange(-2000, 2000, dtype=np.int64)
+        vals = walk.astype(np.uint32).view(np.float32)
+        assert np.array_equal(np.sqrt(vals) <= hf, vals <= T)
+    # h = 0.0457: T is exactly fl32(h*h) (bits 0x3b08df0c)
+    assert np.float32(O.pred_threshold(np.float32(0.0457))).view(np.uint32) == 0x3B08DF0C
+
+
+def test_neighbour_brute_force_matches_reference_predicate():
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-0.1, 0.1, size=(400, 3)).astype(np.float32)
+    thr = O.pred_threshold(H)
+    cnt, off, idx = O.neighbours_brute(pos, thr)
+    d = pos[:, None, :] - pos[None, :, :]
+    d2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    acc = np.sqrt(d2.astype(np.float32)) <= np.float32(H)
+    assert np.array_equal(cnt, acc.sum(1))
+    for i in range(0, 400, 37):
+        assert np.array_equal(idx[off[i]:off[i + 1]], np.nonzero(acc[i])[0])
