@@ -103,6 +103,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--trace", type=int, default=0, help="steps to trace before the warmup (diagnostics)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -119,6 +120,15 @@ def main():
         blk, rm = ctx.autotune(3)
         log(f"bench: autotuned block={blk} xcd_remap={rm}")
     blk, rm = ctx.get_tuning()
+    if a.trace:
+        # ms/step and neighbour counts along the trajectory, 50-step chunks
+        for c in range(0, a.trace, 50):
+            t = time.perf_counter()
+            ctx.step(50)
+            ctx.sync()
+            ms = (time.perf_counter() - t) * 1e3 / 50
+            cnt = ctx.debug_neighbours(sids[0], csr=False)[0]
+            log(f"trace: steps {c + 50:5d} ms/step {ms:.3f} neighbours mean {cnt.mean():.1f} max {cnt.max()}")
     ctx.step(a.warmup)
     ctx.sync()
     mass0 = [ctx.mass(s) for s in sids]
@@ -173,6 +183,14 @@ def main():
     out["mass_drift_rel"] = drift
     out["mass_units_exact"] = units_exact
     out["nonfinite"] = status
+
+    # neighbour statistics of the state the timed region ended in
+    counts = np.concatenate([ctx.debug_neighbours(s, csr=False)[0] for s in sids])
+    rho = np.concatenate([ctx.debug_density(s) for s in sids])
+    rho0 = np.array([s.params.rest_density for s in scs for _ in range(s.n)])
+    out["neighbours"] = dict(mean=float(counts.mean()), p50=float(np.median(counts)),
+                             p99=float(np.percentile(counts, 99)), max=int(counts.max()))
+    out["density_over_rest"] = dict(mean=float((rho / rho0).mean()), max=float((rho / rho0).max()))
 
     if len(scs) == 1 and N <= 300000:
         log("bench: density error vs FP64 oracle")

@@ -228,6 +228,12 @@ int sphe_kernel_bytes(sphe_ctx* ctx, const char* kernel, double* bytes);
  * candidate and keeps the fastest. */
 int sphe_set_tuning(sphe_ctx* ctx, int block, int xcd_remap);
 int sphe_get_tuning(const sphe_ctx* ctx, int* block, int* xcd_remap);
+/* Lanes per particle in the density and force passes (1, 2, 4 or 8;
+ * default 4).  Lane l takes positions l, l+G, ... of each stencil row and the
+ * partial sums are merged in a fixed xor tree, so results are deterministic
+ * for a given lane count (and differ between counts by summation order). */
+int sphe_set_lanes(sphe_ctx* ctx, int lanes);
+int sphe_get_lanes(const sphe_ctx* ctx, int* lanes);
 int sphe_autotune(sphe_ctx* ctx, int steps_per_trial);
 
 /* ---- verification hooks (used by the parity tests) ---------------------- */

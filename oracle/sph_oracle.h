@@ -100,6 +100,7 @@ typedef struct {
     int32_t fx_exp;                 /* fixed-point exponent: 1 unit = 2^-fx_exp volume */
     int32_t erosion;                /* 0 = off, 1 = on */
     double kc, ke, kd, band;        /* capacity coefficient, erode/deposit rates, contact band */
+    int32_t lanes;                  /* lanes per particle splitting the neighbour sums (1, 2, 4, 8) */
 } orc_scene;
 
 /* Derived per-scene constants, identical in meaning to the GPU's. */

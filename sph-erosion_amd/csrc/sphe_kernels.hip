@@ -77,8 +77,13 @@ __device__ __forceinline__ void cell_of(const SceneConsts<T>& K, const int32_t* 
     kz = cidx<T>(d_floor((z - K.org[2]) * K.inv_cell), dims[2] - 1);
 }
 
+// Neighbour-sum accumulators.  A particle's sum is split over the G lanes of
+// its group (lane l takes the candidates at positions l, l+G, ... of each
+// stencil row); merge_xor(m) combines the partials of lanes l and l^m, so a
+// xor tree over m = 1, 2, .. G/2 leaves the whole sum in every lane.
 template <typename T, bool COMP> struct Acc;
 template <typename T> struct Acc<T, true> {
+    // Kahan: the represented sum is s - c
     T s = T(0), c = T(0);
     __device__ __forceinline__ void add(T v)
     {
@@ -87,11 +92,87 @@ template <typename T> struct Acc<T, true> {
         c = (t - s) - y;
         s = t;
     }
+    // TwoSum of the heads (exact error e), compensations added: the result
+    // is the same in both lanes of the pair.
+    __device__ __forceinline__ void merge_xor(int m)
+    {
+        const T os = __shfl_xor(s, m, SPHE_WAVE);
+        const T oc = __shfl_xor(c, m, SPHE_WAVE);
+        const bool low = (threadIdx.x & m) == 0;
+        const T a = low ? s : os, b = low ? os : s;
+        const T ca = low ? c : oc, cb = low ? oc : c;
+        const T sum = a + b;
+        const T bb = sum - a;
+        const T e = (a - (sum - bb)) + (b - bb);
+        s = sum;
+        c = (ca + cb) - e;
+    }
+    __device__ __forceinline__ T value() const { return s - c; }
 };
 template <typename T> struct Acc<T, false> {
     T s = T(0);
     __device__ __forceinline__ void add(T v) { s = s + v; }
+    __device__ __forceinline__ void merge_xor(int m)
+    {
+        const T os = __shfl_xor(s, m, SPHE_WAVE);
+        const bool low = (threadIdx.x & m) == 0;
+        s = low ? s + os : os + s;
+    }
+    __device__ __forceinline__ T value() const { return s; }
 };
+
+template <int G, typename A>
+__device__ __forceinline__ void group_merge(A& a)
+{
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) a.merge_xor(m);
+}
+
+// Stencil rows of one particle for lane l of its G-lane group: in every row
+// the lane takes positions l, l+G, l+2G, ... (two loads in flight) and calls
+// f(j, dx, dy, dz, d2) for those inside h.  Rows go dz outer, dy inner.
+template <typename T, int G, typename F>
+__device__ __forceinline__ void visit_strided(const SceneDev& S, const SceneConsts<T>& K,
+                                              const uint32_t* __restrict__ start,
+                                              const V4<T>* __restrict__ pos, T xi, T yi, T zi,
+                                              int kx, int ky, int kz, int l, F&& f)
+{
+    const int nx = S.dims[0], ny = S.dims[1], nz = S.dims[2];
+    const int x0 = kx > 0 ? kx - 1 : 0;
+    const int x1 = kx < nx - 1 ? kx + 1 : nx - 1;
+    const uint32_t lo = (uint32_t)S.p_begin, hi = (uint32_t)(S.p_begin + S.n);
+    const T thr = K.thr;
+    for (int dzc = -1; dzc <= 1; ++dzc) {
+        const int z = kz + dzc;
+        if (z < 0 || z >= nz) continue;
+        for (int dyc = -1; dyc <= 1; ++dyc) {
+            const int y = ky + dyc;
+            if (y < 0 || y >= ny) continue;
+            const int64_t rk = (int64_t)S.cell_off + ((int64_t)z * ny + y) * nx;
+            // rows of real cells lie inside [p_begin, p_begin + n); the clamp
+            // only matters if the counts were inconsistent
+            const uint32_t j0 = max(start[rk + x0], lo);
+            const uint32_t j1 = min(start[rk + x1 + 1], hi);
+            uint32_t j = j0 + (uint32_t)l;
+            for (; j + G < j1; j += 2 * G) {
+                const V4<T> pa = pos[j];
+                const V4<T> pb = pos[j + G];
+                const T ax = xi - pa.x, ay = yi - pa.y, az = zi - pa.z;
+                const T a2 = (ax * ax + ay * ay) + az * az;
+                if (a2 <= thr) f(j, ax, ay, az, a2);
+                const T bx = xi - pb.x, by = yi - pb.y, bz = zi - pb.z;
+                const T b2 = (bx * bx + by * by) + bz * bz;
+                if (b2 <= thr) f(j + G, bx, by, bz, b2);
+            }
+            if (j < j1) {
+                const V4<T> pj = pos[j];
+                const T dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+                const T d2 = (dx * dx + dy * dy) + dz * dz;
+                if (d2 <= thr) f(j, dx, dy, dz, d2);
+            }
+        }
+    }
+}
 
 // Visits the candidates of the 27-cell stencil of (kx, ky, kz) in ascending
 // slot order (rows dz, dy; x-cells contiguous inside a row) and calls
@@ -117,7 +198,18 @@ __device__ __forceinline__ void visit_neighbours(const SceneDev& S, const SceneC
             // only matters if the counts were inconsistent
             const uint32_t j0 = max(start[rk + x0], lo);
             const uint32_t j1 = min(start[rk + x1 + 1], hi);
-            for (uint32_t j = j0; j < j1; ++j) {
+            uint32_t j = j0;
+            for (; j + 1 < j1; j += 2) {  // two loads in flight
+                const V4<T> pa = pos[j];
+                const V4<T> pb = pos[j + 1];
+                const T ax = xi - pa.x, ay = yi - pa.y, az = zi - pa.z;
+                const T a2 = (ax * ax + ay * ay) + az * az;
+                if (a2 <= K.thr) f(j, ax, ay, az, a2);
+                const T bx = xi - pb.x, by = yi - pb.y, bz = zi - pb.z;
+                const T b2 = (bx * bx + by * by) + bz * bz;
+                if (b2 <= K.thr) f(j + 1, bx, by, bz, b2);
+            }
+            if (j < j1) {
                 const V4<T> pj = pos[j];
                 const T dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
                 const T d2 = (dx * dx + dy * dy) + dz * dz;
@@ -284,14 +376,21 @@ __global__ __launch_bounds__(256) void k_reorder(DevState D)
 
 // --------------------------------------------------------------- density
 
-template <typename T, bool COMP>
+// One G-lane group per particle (lane l = thread index mod G).  Each lane
+// sums its share of the stencil and writes its accepted neighbours to its own
+// sub-list (nbl[slot * maxn + k * G + l], k < maxn / G; its count in
+// nbc[slot * G + l]) for the force pass; the group then merges the partial
+// sums and lane 0 writes rho and (p / rho^2, m / rho).
+template <typename T, bool COMP, int G>
 __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap)
 {
     const int b = xcd_block(blockIdx.x, nblk, remap);
-    const int64_t slot = (int64_t)b * blockDim.x + threadIdx.x;
+    const int64_t gt = (int64_t)b * blockDim.x + threadIdx.x;
+    const int64_t slot = gt / G;
+    const int l = (int)(gt & (G - 1));
     if (slot >= D.P) return;
     const SceneDev& S = D.scenes[wave_scene(D.chunk_scene, slot)];
-    if (slot >= S.p_begin + S.n) return;
+    if (slot >= S.p_begin + S.n) return;  // whole groups leave together
     const SceneConsts<T>& K = SceneSel<T>::get(S);
     const V4<T>* __restrict__ pos = (const V4<T>*)D.s_pos;
     const V4<T> pi = pos[slot];
@@ -299,13 +398,22 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     cell_of<T>(K, S.dims, pi.x, pi.y, pi.z, kx, ky, kz);
     Acc<T, COMP> sum;
     const T h2 = K.h2;
-    visit_neighbours<T>(S, K, D.start, pos, pi.x, pi.y, pi.z, kx, ky, kz,
-                        [&](uint32_t, T, T, T, T d2) {
+    uint32_t cnt = 0;
+    const uint32_t cap = (uint32_t)D.maxn / G;
+    uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn + l;
+    visit_strided<T, G>(S, K, D.start, pos, pi.x, pi.y, pi.z, kx, ky, kz, l,
+                        [&](uint32_t j, T, T, T, T d2) {
                             T q = h2 - d2;
                             q = q > T(0) ? q : T(0);
                             sum.add((q * q) * q);
+                            if (cnt < cap) nbl[(int64_t)cnt * G] = j;
+                            ++cnt;
                         });
-    const T rho = sum.s * K.mc6;
+    D.nbc[slot * G + l] = cnt;
+    if (cnt > cap) atomicMax(D.ovf, cnt * (uint32_t)G);
+    group_merge<G>(sum);
+    if (l != 0) return;
+    const T rho = sum.value() * K.mc6;
     const T p = K.k * (rho - K.rho0);
     ((T*)D.rho)[slot] = rho;
     V2<T> a;
@@ -324,73 +432,97 @@ struct ForceOut {
 };
 
 // Pressure, viscosity, colour normal and colour Laplacian sums of one
-// particle (fluid_system.h:136-150, 165-172) with the constants factored out.
-template <typename T, bool COMP, bool INSPECT>
-__device__ __forceinline__ void force_sums(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
-                                           int64_t slot, const V4<T>& pi, const V4<T>& vi, T rho_i, T a_i,
-                                           ForceOut<T>& o)
+// particle (fluid_system.h:136-150, 165-172) with the constants factored out
+// of the sums.  pair() takes one neighbour (self included) in canonical
+// order; finish() applies the constants and forms the acceleration
+// (fluid_system.h:151-154, 163, 177, 318-321).
+template <typename T, bool COMP>
+struct ForceAcc {
+    Acc<T, COMP> fp0, fp1, fp2, fv0, fv1, fv2, nn0, nn1, nn2, lap;
+
+    __device__ __forceinline__ void pair(const SceneConsts<T>& K, bool self, T dx, T dy, T dz, T d2, T a_i, T a_j,
+                                         T b_j, const V4<T>& vi, const V4<T>& vj)
+    {
+        T q = K.h2 - d2;
+        q = q > T(0) ? q : T(0);
+        lap.add(b_j * (q * (K.t3h2 - T(7) * d2)));
+        if (self) return;
+        const T r = d_sqrt(d2);
+        const T hr = K.h - r;
+        T ux, uy, uz;
+        if (r < K.eps_r) {
+            // coincident pair: the reference points along normalize(1,1,1)
+            // (fluid_system.h:438-440)
+            ux = K.inv_sqrt3; uy = K.inv_sqrt3; uz = K.inv_sqrt3;
+        } else {
+            const T inv = T(1) / r;
+            ux = dx * inv; uy = dy * inv; uz = dz * inv;
+        }
+        const T sp = (a_i + a_j) * (hr * hr);
+        fp0.add(sp * ux);
+        fp1.add(sp * uy);
+        fp2.add(sp * uz);
+        const T wv = b_j * hr;
+        fv0.add((vj.x - vi.x) * wv);
+        fv1.add((vj.y - vi.y) * wv);
+        fv2.add((vj.z - vi.z) * wv);
+        const T wn = b_j * (q * q);
+        nn0.add(dx * wn);
+        nn1.add(dy * wn);
+        nn2.add(dz * wn);
+    }
+
+    template <int G>
+    __device__ __forceinline__ void merge()
+    {
+        group_merge<G>(fp0); group_merge<G>(fp1); group_merge<G>(fp2);
+        group_merge<G>(fv0); group_merge<G>(fv1); group_merge<G>(fv2);
+        group_merge<G>(nn0); group_merge<G>(nn1); group_merge<G>(nn2);
+        group_merge<G>(lap);
+    }
+
+    __device__ __forceinline__ void finish(const SceneConsts<T>& K, T rho_i, ForceOut<T>& o) const
+    {
+        const T sPF = K.pf_scale * rho_i;
+        o.pf[0] = fp0.value() * sPF; o.pf[1] = fp1.value() * sPF; o.pf[2] = fp2.value() * sPF;
+        o.vf[0] = fv0.value() * K.vf_scale; o.vf[1] = fv1.value() * K.vf_scale; o.vf[2] = fv2.value() * K.vf_scale;
+        o.nr[0] = nn0.value() * K.ng; o.nr[1] = nn1.value() * K.ng; o.nr[2] = nn2.value() * K.ng;
+        const T lapv = lap.value() * K.ng;
+        const T ss = (-K.surf_tens) * lapv;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            o.sf[a] = o.nr[a] * ss;
+            o.gf[a] = rho_i * K.g[a];
+            const T F = (o.pf[a] + o.vf[a]) + (o.gf[a] + o.sf[a]);
+            o.acc[a] = F / rho_i;
+        }
+    }
+};
+
+// Lane l's share of the force sums of one particle by walking its stencil
+// positions (inspection, and particles whose sub-list overflowed), with the
+// largest neighbour id and the neighbour count of that share.
+template <typename T, bool COMP, int G>
+__device__ __forceinline__ void force_share_walk(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
+                                                 int64_t slot, int l, const V4<T>& pi, const V4<T>& vi, T a_i,
+                                                 ForceAcc<T, COMP>& acc, int32_t& nbmax, int32_t& count)
 {
     const V4<T>* __restrict__ pos = (const V4<T>*)D.s_pos;
     const V4<T>* __restrict__ vel = (const V4<T>*)D.s_vel;
     const V2<T>* __restrict__ aux = (const V2<T>*)D.aux;
     int kx, ky, kz;
     cell_of<T>(K, S.dims, pi.x, pi.y, pi.z, kx, ky, kz);
-    Acc<T, COMP> fp0, fp1, fp2, fv0, fv1, fv2, nn0, nn1, nn2, lap;
-    int32_t nbmax = -1, count = 0;
-    const T h = K.h, h2 = K.h2, t3h2 = K.t3h2, eps = K.eps_r, is3 = K.inv_sqrt3;
-    visit_neighbours<T>(S, K, D.start, pos, pi.x, pi.y, pi.z, kx, ky, kz,
+    visit_strided<T, G>(S, K, D.start, pos, pi.x, pi.y, pi.z, kx, ky, kz, l,
                         [&](uint32_t j, T dx, T dy, T dz, T d2) {
-                            T q = h2 - d2;
-                            q = q > T(0) ? q : T(0);
+                            const bool self = (int64_t)j == slot;
+                            ++count;
+                            if (!self) {
+                                const int32_t idj = D.s_id[j];
+                                nbmax = idj > nbmax ? idj : nbmax;
+                            }
                             const V2<T> aj = aux[j];
-                            const T bj = aj.y;
-                            lap.add(bj * (q * (t3h2 - T(7) * d2)));
-                            if (INSPECT) {
-                                ++count;
-                                if ((int64_t)j != slot) {
-                                    const int32_t idj = D.s_id[j];
-                                    nbmax = idj > nbmax ? idj : nbmax;
-                                }
-                            }
-                            if ((int64_t)j == slot) return;
-                            const T r = d_sqrt(d2);
-                            const T hr = h - r;
-                            T ux, uy, uz;
-                            if (r < eps) {
-                                ux = is3; uy = is3; uz = is3;
-                            } else {
-                                const T inv = T(1) / r;
-                                ux = dx * inv; uy = dy * inv; uz = dz * inv;
-                            }
-                            const T sp = (a_i + aj.x) * (hr * hr);
-                            fp0.add(sp * ux);
-                            fp1.add(sp * uy);
-                            fp2.add(sp * uz);
-                            const V4<T> vj = vel[j];
-                            const T wv = bj * hr;
-                            fv0.add((vj.x - vi.x) * wv);
-                            fv1.add((vj.y - vi.y) * wv);
-                            fv2.add((vj.z - vi.z) * wv);
-                            const T wn = bj * (q * q);
-                            nn0.add(dx * wn);
-                            nn1.add(dy * wn);
-                            nn2.add(dz * wn);
+                            acc.pair(K, self, dx, dy, dz, d2, a_i, aj.x, aj.y, vi, vel[j]);
                         });
-    const T sPF = K.pf_scale * rho_i;
-    o.pf[0] = fp0.s * sPF; o.pf[1] = fp1.s * sPF; o.pf[2] = fp2.s * sPF;
-    o.vf[0] = fv0.s * K.vf_scale; o.vf[1] = fv1.s * K.vf_scale; o.vf[2] = fv2.s * K.vf_scale;
-    o.nr[0] = nn0.s * K.ng; o.nr[1] = nn1.s * K.ng; o.nr[2] = nn2.s * K.ng;
-    const T lapv = lap.s * K.ng;
-    const T ss = (-K.surf_tens) * lapv;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        o.sf[a] = o.nr[a] * ss;
-        o.gf[a] = rho_i * K.g[a];
-        const T F = (o.pf[a] + o.vf[a]) + (o.gf[a] + o.sf[a]);
-        o.acc[a] = F / rho_i;
-    }
-    o.nbmax = nbmax;
-    o.count = count;
 }
 
 // Dominant-axis box response (fluid_system.h:342-347 with collisionS
@@ -501,11 +633,63 @@ __device__ __forceinline__ int64_t erode_exchange(const SceneDev& S, const Scene
     return dq;
 }
 
-template <typename T, bool COMP>
+// Lane l's share of the force sums from the sub-list the density pass wrote
+// (its stencil positions l, l+G, ... that lie inside h, in order, self
+// included).  Only ~1 in 8 stencil candidates lies inside h, so walking the
+// stencil again would run the ~100-instruction pair body for the whole wave
+// whenever any lane accepts; the list holds accepted pairs only.  Entries are
+// read two ahead and neighbour data one ahead so the dependent gathers overlap
+// the pair arithmetic.  A lane whose share overflowed the list walks its
+// stencil positions instead (same pairs, same order, same result).
+template <typename T, bool COMP, int G>
+__device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
+                                           int64_t slot, int l, const V4<T>& pi, const V4<T>& vi, T a_i,
+                                           ForceAcc<T, COMP>& acc)
+{
+    const uint32_t cnt = D.nbc[slot * G + l];
+    if (cnt > (uint32_t)D.maxn / G) {
+        int32_t nbmax = -1, count = 0;
+        force_share_walk<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc, nbmax, count);
+        return;
+    }
+    if (cnt == 0) return;
+    const V4<T>* __restrict__ pos = (const V4<T>*)D.s_pos;
+    const V4<T>* __restrict__ vel = (const V4<T>*)D.s_vel;
+    const V2<T>* __restrict__ aux = (const V2<T>*)D.aux;
+    const uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn + l;
+    uint32_t jn = nbl[0];
+    uint32_t jn2 = cnt > 1 ? nbl[G] : jn;
+    V4<T> pn = pos[jn];
+    V4<T> vn = vel[jn];
+    V2<T> an = aux[jn];
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const uint32_t j = jn;
+        const V4<T> pj = pn;
+        const V4<T> vj = vn;
+        const V2<T> aj = an;
+        jn = jn2;
+        if (k + 2 < cnt) jn2 = nbl[(k + 2) * G];
+        if (k + 1 < cnt) {
+            pn = pos[jn];
+            vn = vel[jn];
+            an = aux[jn];
+        }
+        const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const T d2 = (dx * dx + dy * dy) + dz * dz;
+        acc.pair(K, (int64_t)j == slot, dx, dy, dz, d2, a_i, aj.x, aj.y, vi, vj);
+    }
+}
+
+// One G-lane group per particle: the lanes sum their shares, merge them
+// (xor shuffles), and lane 0 integrates, collides, erodes, stores the new
+// state and counts the particle's next cell.
+template <typename T, bool COMP, int G>
 __global__ __launch_bounds__(512) void k_force(DevState D, int nblk, int remap)
 {
     const int b = xcd_block(blockIdx.x, nblk, remap);
-    const int64_t slot = (int64_t)b * blockDim.x + threadIdx.x;
+    const int64_t gt = (int64_t)b * blockDim.x + threadIdx.x;
+    const int64_t slot = gt / G;
+    const int l = (int)(gt & (G - 1));
     if (slot >= D.P) return;
     const SceneDev& S = D.scenes[wave_scene(D.chunk_scene, slot)];
     V4<T>* __restrict__ upos = (V4<T>*)D.u_pos;
@@ -513,6 +697,7 @@ __global__ __launch_bounds__(512) void k_force(DevState D, int nblk, int remap)
     const V4<T>* __restrict__ spos = (const V4<T>*)D.s_pos;
     const V4<T>* __restrict__ svel = (const V4<T>*)D.s_vel;
     if (slot >= S.p_begin + S.n) {
+        if (l != 0) return;
         // padding: carried through, counted into the scene's pad cell
         upos[slot] = spos[slot];
         uvel[slot] = svel[slot];
@@ -529,8 +714,12 @@ __global__ __launch_bounds__(512) void k_force(DevState D, int nblk, int remap)
     const V4<T> vi = svel[slot];
     const T rho_i = ((const T*)D.rho)[slot];
     const T a_i = ((const V2<T>*)D.aux)[slot].x;
+    ForceAcc<T, COMP> acc;
+    force_share<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc);
+    acc.template merge<G>();
+    if (l != 0) return;
     ForceOut<T> o;
-    force_sums<T, COMP, false>(D, S, K, slot, pi, vi, rho_i, a_i, o);
+    acc.finish(K, rho_i, o);
 
     // symplectic Euler (fluid_system.h:330-331)
     T p[3], v[3];
@@ -630,18 +819,34 @@ __global__ __launch_bounds__(256) void k_tapply(DevState D)
 
 // ----------------------------------------------------------- inspection
 
-template <typename T, bool COMP>
+// The record of one particle after a step: its group recomputes the force
+// sums of that step (same lane split and merge as k_force) from the retained
+// sorted state.
+template <typename T, bool COMP, int G>
 __global__ void k_inspect(DevState D, int64_t slot, double* out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (blockIdx.x != 0 || threadIdx.x >= G) return;
+    const int l = (int)threadIdx.x;
     const SceneDev& S = D.scenes[D.chunk_scene[slot >> 6]];
     const SceneConsts<T>& K = SceneSel<T>::get(S);
     const V4<T> pi = ((const V4<T>*)D.s_pos)[slot];
     const V4<T> vi = ((const V4<T>*)D.s_vel)[slot];
     const T rho_i = ((const T*)D.rho)[slot];
     const T a_i = ((const V2<T>*)D.aux)[slot].x;
+    ForceAcc<T, COMP> acc;
+    int32_t nbmax = -1, count = 0;
+    force_share_walk<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc, nbmax, count);
+    acc.template merge<G>();
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) {
+        nbmax = max(nbmax, __shfl_xor(nbmax, m, SPHE_WAVE));
+        count += __shfl_xor(count, m, SPHE_WAVE);
+    }
+    if (l != 0) return;
     ForceOut<T> o;
-    force_sums<T, COMP, true>(D, S, K, slot, pi, vi, rho_i, a_i, o);
+    acc.finish(K, rho_i, o);
+    o.nbmax = nbmax;
+    o.count = count;
     out[0] = (double)rho_i;
     out[1] = (double)(K.k * (rho_i - K.rho0));
     for (int a = 0; a < 3; ++a) {
@@ -804,24 +1009,46 @@ void sphe_launch_reorder(const DevState& D, hipStream_t st)
     else k_reorder<float><<<grid_of(D.P, 256), 256, 0, st>>>(D);
 }
 
-void sphe_launch_density(const DevState& D, int block, int remap, hipStream_t st)
+template <int G>
+static void launch_density_g(const DevState& D, int block, int remap, hipStream_t st)
 {
-    if (D.P == 0) return;
-    const int nblk = (int)grid_of(D.P, block);
+    const int nblk = (int)grid_of(D.P * G, block);
     SPHE_PREC_SWITCH(D.precision,
-                     (k_density<float, true><<<nblk, block, 0, st>>>(D, nblk, remap)),
-                     (k_density<float, false><<<nblk, block, 0, st>>>(D, nblk, remap)),
-                     (k_density<double, false><<<nblk, block, 0, st>>>(D, nblk, remap)));
+                     (k_density<float, true, G><<<nblk, block, 0, st>>>(D, nblk, remap)),
+                     (k_density<float, false, G><<<nblk, block, 0, st>>>(D, nblk, remap)),
+                     (k_density<double, false, G><<<nblk, block, 0, st>>>(D, nblk, remap)));
 }
 
-void sphe_launch_force(const DevState& D, int block, int remap, hipStream_t st)
+template <int G>
+static void launch_force_g(const DevState& D, int block, int remap, hipStream_t st)
+{
+    const int nblk = (int)grid_of(D.P * G, block);
+    SPHE_PREC_SWITCH(D.precision,
+                     (k_force<float, true, G><<<nblk, block, 0, st>>>(D, nblk, remap)),
+                     (k_force<float, false, G><<<nblk, block, 0, st>>>(D, nblk, remap)),
+                     (k_force<double, false, G><<<nblk, block, 0, st>>>(D, nblk, remap)));
+}
+
+void sphe_launch_density(const DevState& D, int block, int remap, int lanes, hipStream_t st)
 {
     if (D.P == 0) return;
-    const int nblk = (int)grid_of(D.P, block);
-    SPHE_PREC_SWITCH(D.precision,
-                     (k_force<float, true><<<nblk, block, 0, st>>>(D, nblk, remap)),
-                     (k_force<float, false><<<nblk, block, 0, st>>>(D, nblk, remap)),
-                     (k_force<double, false><<<nblk, block, 0, st>>>(D, nblk, remap)));
+    switch (lanes) {
+    case 1: launch_density_g<1>(D, block, remap, st); break;
+    case 2: launch_density_g<2>(D, block, remap, st); break;
+    case 8: launch_density_g<8>(D, block, remap, st); break;
+    default: launch_density_g<4>(D, block, remap, st); break;
+    }
+}
+
+void sphe_launch_force(const DevState& D, int block, int remap, int lanes, hipStream_t st)
+{
+    if (D.P == 0) return;
+    switch (lanes) {
+    case 1: launch_force_g<1>(D, block, remap, st); break;
+    case 2: launch_force_g<2>(D, block, remap, st); break;
+    case 8: launch_force_g<8>(D, block, remap, st); break;
+    default: launch_force_g<4>(D, block, remap, st); break;
+    }
 }
 
 void sphe_launch_heights(const DevState& D, hipStream_t st)
@@ -841,12 +1068,23 @@ void sphe_launch_two_phase(const DevState& D, hipStream_t st)
 
 void sphe_launch_clear_status(const DevState& D, hipStream_t st) { k_clear_status<<<1, 64, 0, st>>>(D.status); }
 
-void sphe_launch_inspect(const DevState& D, int64_t slot, double* out, hipStream_t st)
+template <int G>
+static void launch_inspect_g(const DevState& D, int64_t slot, double* out, hipStream_t st)
 {
     SPHE_PREC_SWITCH(D.precision,
-                     (k_inspect<float, true><<<1, 64, 0, st>>>(D, slot, out)),
-                     (k_inspect<float, false><<<1, 64, 0, st>>>(D, slot, out)),
-                     (k_inspect<double, false><<<1, 64, 0, st>>>(D, slot, out)));
+                     (k_inspect<float, true, G><<<1, 64, 0, st>>>(D, slot, out)),
+                     (k_inspect<float, false, G><<<1, 64, 0, st>>>(D, slot, out)),
+                     (k_inspect<double, false, G><<<1, 64, 0, st>>>(D, slot, out)));
+}
+
+void sphe_launch_inspect(const DevState& D, int64_t slot, int lanes, double* out, hipStream_t st)
+{
+    switch (lanes) {
+    case 1: launch_inspect_g<1>(D, slot, out, st); break;
+    case 2: launch_inspect_g<2>(D, slot, out, st); break;
+    case 8: launch_inspect_g<8>(D, slot, out, st); break;
+    default: launch_inspect_g<4>(D, slot, out, st); break;
+    }
 }
 
 void sphe_launch_neighbours(const DevState& D, int scene, const int64_t* off, int32_t* counts, int32_t* idx,

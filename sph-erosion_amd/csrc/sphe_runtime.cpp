@@ -88,6 +88,8 @@ struct sphe_ctx {
 
     int block = 256;
     int remap = 1;
+    int maxn = 128;  // neighbour-list capacity per particle (grown when a step overflows it)
+    int lanes = 4;   // lanes per particle in the density and force passes
 
     double* d_mass_part = nullptr;
     double* d_mass_out = nullptr;
@@ -381,6 +383,10 @@ int build_layout(sphe_ctx* c)
     D.ntpartial = sphe_scan_partials(TC + 1);
     DALLOC(D.tpartial, (size_t)D.ntpartial * 4);
     DALLOC(D.tdq, Pn * 8);
+    D.maxn = c->maxn;
+    DALLOC(D.nbl, Pn * (size_t)D.maxn * 4);
+    DALLOC(D.nbc, Pn * 8 * 4);  // up to 8 lanes per particle
+    DALLOC(D.ovf, 64);
     DALLOC(D.status, 64);
     DALLOC(c->d_mass_part, (size_t)sphe_mass_blocks() * 3 * 8);
     DALLOC(c->d_mass_out, 64);
@@ -406,6 +412,7 @@ int build_layout(sphe_ctx* c)
     HIPCHK(c, hipMemsetAsync(D.cnt, 0, (size_t)(C + 1) * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(D.tcnt, 0, (size_t)(TC + 1) * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(D.status, 0, 64, c->stream));
+    HIPCHK(c, hipMemsetAsync(D.ovf, 0, 64, c->stream));
     if (TC > 0) HIPCHK(c, hipMemsetAsync(D.vol, 0, (size_t)TC * 8, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->layout_valid = true;
@@ -657,11 +664,11 @@ void launch_step(sphe_ctx* c)
     launch_sort(c);
     {
         Timed t(c, "density");
-        sphe_launch_density(D, c->block, c->remap, c->stream);
+        sphe_launch_density(D, c->block, c->remap, c->lanes, c->stream);
     }
     {
         Timed t(c, "force");
-        sphe_launch_force(D, c->block, c->remap, c->stream);
+        sphe_launch_force(D, c->block, c->remap, c->lanes, c->stream);
     }
     if (any_erosion(c, SPHE_EROSION_TWO_PHASE)) {
         Timed t(c, "two_phase");
@@ -676,9 +683,10 @@ void launch_step(sphe_ctx* c)
 std::string plan_signature(const sphe_ctx* c)
 {
     char b[256];
-    snprintf(b, sizeof(b), "%d/%d/%d/%lld/%lld/%lld/%d/%d/%p", c->precision, c->block, c->remap,
-             (long long)c->D.P, (long long)c->D.C, (long long)c->D.TC, (int)any_erosion(c, -1),
-             (int)any_erosion(c, SPHE_EROSION_TWO_PHASE), (const void*)c->D.u_pos);
+    snprintf(b, sizeof(b), "%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%d", c->precision, c->block, c->remap,
+             c->lanes, (long long)c->D.P, (long long)c->D.C, (long long)c->D.TC, (int)any_erosion(c, -1),
+             (int)any_erosion(c, SPHE_EROSION_TWO_PHASE), (const void*)c->D.u_pos, (const void*)c->D.nbl,
+             c->D.maxn);
     return b;
 }
 
@@ -714,11 +722,42 @@ int recount(sphe_ctx* c)
     return SPHE_OK;
 }
 
+// The neighbour lists hold maxn entries per particle.  When a step found a
+// larger count (those lanes walked their stencil again, with the same
+// result), the lists grow for the following steps.
+int grow_lists(sphe_ctx* c)
+{
+    uint32_t need = 0;
+    HIPCHK(c, hipMemcpyAsync(&need, c->D.ovf, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (need <= (uint32_t)c->D.maxn) return SPHE_OK;
+    const int step = 16 * 8;
+    const int maxn = (int)(((int64_t)need * 5 / 4 + step - 1) / step * step);
+    void* p = nullptr;
+    const size_t bytes = (size_t)c->D.P * (size_t)maxn * 4;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return SPHE_OK;  // keep the walk for the overflowing lanes
+    for (void*& a : c->allocs)
+        if (a == (void*)c->D.nbl) {
+            (void)hipFree(a);
+            a = p;
+        }
+    c->D.nbl = (uint32_t*)p;
+    c->D.maxn = maxn;
+    c->maxn = maxn;
+    HIPCHK(c, hipMemsetAsync(c->D.ovf, 0, 4, c->stream));
+    return SPHE_OK;
+}
+
 int do_steps(sphe_ctx* c, int nsteps)
 {
     int r = prepare(c);
     if (r) return r;
     if (c->D.P == 0) return SPHE_OK;
+    if (c->stepped) {
+        r = grow_lists(c);
+        if (r) return r;
+    }
     sphe_launch_clear_status(c->D, c->stream);
     if (!c->counted) {
         r = recount(c);
@@ -768,7 +807,7 @@ int sort_current(sphe_ctx* c)
         if (r) return r;
     }
     launch_sort(c);
-    sphe_launch_density(c->D, c->block, c->remap, c->stream);
+    sphe_launch_density(c->D, c->block, c->remap, c->lanes, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->counted = false;
@@ -1183,7 +1222,7 @@ int sphe_get_particle(sphe_ctx* c, int scene, int id, sphe_particle* out)
     for (size_t i = 0; i < n; ++i)
         if (sid[i] == id) { slot = i; break; }
     if (slot == n) return fail(c, SPHE_ERANGE, "id %d not in the last step", id);
-    sphe_launch_inspect(c->D, s->p_begin + (int64_t)slot, c->d_inspect, c->stream);
+    sphe_launch_inspect(c->D, s->p_begin + (int64_t)slot, c->lanes, c->d_inspect, c->stream);
     HIPCHK(c, hipGetLastError());
     double v[22];
     HIPCHK(c, hipMemcpyAsync(v, c->d_inspect, sizeof(v), hipMemcpyDeviceToHost, c->stream));
@@ -1486,6 +1525,21 @@ int sphe_set_tuning(sphe_ctx* c, int block, int remap)
         return fail(c, SPHE_EARG, "block must be 64, 128, 256 or 512");
     c->block = block;
     c->remap = remap ? 1 : 0;
+    return SPHE_OK;
+}
+
+int sphe_set_lanes(sphe_ctx* c, int lanes)
+{
+    if (!c) return SPHE_EARG;
+    if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) return fail(c, SPHE_EARG, "lanes must be 1, 2, 4 or 8");
+    c->lanes = lanes;
+    return SPHE_OK;
+}
+
+int sphe_get_lanes(const sphe_ctx* c, int* lanes)
+{
+    if (!c || !lanes) return SPHE_EARG;
+    *lanes = c->lanes;
     return SPHE_OK;
 }
 

@@ -93,6 +93,8 @@ SIGNATURES = {
     "sphe_set_tuning": (I, [P, I, I]),
     "sphe_get_tuning": (I, [P, C.POINTER(I), C.POINTER(I)]),
     "sphe_autotune": (I, [P, I]),
+    "sphe_set_lanes": (I, [P, I]),
+    "sphe_get_lanes": (I, [P, C.POINTER(I)]),
     "sphe_debug_cell_keys": (I, [P, I, P]),
     "sphe_debug_consts": (I, [P, I, P, I]),
     "sphe_debug_neighbours": (I, [P, I, P, P, P]),

@@ -250,6 +250,14 @@ class Context:
         self._chk(lib().sphe_get_tuning(self._p, C.byref(b), C.byref(r)), "get_tuning")
         return b.value, r.value
 
+    def set_lanes(self, lanes):
+        self._chk(lib().sphe_set_lanes(self._p, lanes), "set_lanes")
+
+    def get_lanes(self):
+        v = C.c_int()
+        self._chk(lib().sphe_get_lanes(self._p, C.byref(v)), "get_lanes")
+        return v.value
+
     def autotune(self, steps_per_trial=5):
         self._chk(lib().sphe_autotune(self._p, steps_per_trial), "autotune")
         return self.get_tuning()

@@ -235,6 +235,49 @@ def test_launch_shape_does_not_change_results(block, remap):
     assert np.array_equal(p1, p2)
 
 
+@pytest.mark.parametrize("lanes", [1, 2, 8])
+def test_lane_split_bit_exact_twin(lanes):
+    """Each lane count is its own summation order; the twin with the same
+    split reproduces the GPU step bit for bit."""
+    sc = small_dam()
+    ctx, sid = gpu_context(sc)
+    ctx.set_lanes(lanes)
+    pos, vel, sed, vol, height, fx = oracle_inputs(ctx, sid, sc)
+    osc = orc_scene_from(sc.params, sc.terrain, fx)
+    osc.lanes = lanes
+    ctx.step(2)
+    gp, gv = ctx.state_f64(sid)
+    st = dict(pos=pos, vel=vel, sed=sed, vol=vol, height=height)
+    for _ in range(2):
+        st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], sc.ids, st["vol"], st["height"], real=4, comp=1)
+    assert np.array_equal(gp, st["pos"]) and np.array_equal(gv, st["vel"])
+    assert np.array_equal(ctx.sediment_fx(sid), st["sed"])
+
+
+def test_list_overflow_walks_the_same_pairs():
+    """A pile-up beyond the neighbour-list capacity: the overflowing lanes walk
+    their stencil positions again, with the same result as the twin; the next
+    steps run with grown lists."""
+    rng = np.random.default_rng(4)
+    n = 3000
+    pos = (0.5 + rng.normal(0, 0.012, size=(n, 3))).astype(np.float32)  # dense blob
+    p = sphe.default_params(h=0.03, mass=0.001, rest_density=1000.0, dt=1e-4, box_center=(0.5, 0.5, 0.5),
+                            box_half=0.5, gravity=(0.0, -9.8, 0.0))
+    ctx = sphe.Context()
+    sid = ctx.add_scene(p)
+    ctx.set_particles(sid, pos)
+    counts = ctx.debug_neighbours(sid, csr=False)[0]
+    assert counts.max() > 128  # beyond the initial list capacity
+    osc = orc_scene_from(p)
+    ctx.step(3)
+    st = dict(pos=pos.astype(np.float64), vel=np.zeros((n, 3)), sed=np.zeros(n, np.int64))
+    for _ in range(3):
+        st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], np.arange(n, dtype=np.int32), real=4, comp=1)
+    gp, gv = ctx.state_f64(sid)
+    assert ctx.status() == 0
+    assert np.array_equal(gp, st["pos"]) and np.array_equal(gv, st["vel"])
+
+
 def test_status_flags_nonfinite():
     # a particle at rest outside the box: the reference's response divides by
     # |v| = 0 and yields NaN velocity (fluid_system.h:345)
