@@ -104,12 +104,24 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--trace", type=int, default=0, help="steps to trace before the warmup (diagnostics)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the timed region (for profiler runs): no statistics, oracle, FP64 or CPU legs")
+    ap.add_argument("--mode", default="step", choices=["step", "copy"],
+                    help="copy: time a plain 1 GiB device copy (HBM reference and counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     import sphe
+    if a.mode == "copy":
+        ctx = sphe.Context(sphe.FP32)
+        nbytes = 1 << 30
+        ms = ctx.debug_copy(nbytes, 20)
+        print(json.dumps(dict(mode="copy", bytes=nbytes, avg_ms=ms, gbs=2 * nbytes / (ms * 1e-3) / 1e9)), flush=True)
+        return
+    if a.no_extras:
+        a.no_cpu = a.no_fp64 = a.no_tune = True
     log(f"bench: {sphe.lib().sphe_version().decode()} workload {a.workload}")
     scs, desc = make_workload(a.workload)
     N = sum(s.n for s in scs)
@@ -184,6 +196,9 @@ def main():
     out["mass_units_exact"] = units_exact
     out["nonfinite"] = status
 
+    if a.no_extras:
+        print(json.dumps(out), flush=True)
+        return
     # neighbour statistics of the state the timed region ended in
     counts = np.concatenate([ctx.debug_neighbours(s, csr=False)[0] for s in sids])
     rho = np.concatenate([ctx.debug_density(s) for s in sids])

@@ -251,6 +251,11 @@ int sphe_debug_neighbours(sphe_ctx* ctx, int scene, int32_t* counts, const int64
                           int32_t* idx);
 /* Density of the current state (one density pass, no step), id order. */
 int sphe_debug_density(sphe_ctx* ctx, int scene, double* rho);
+/* Plain device-to-device streaming copy of `bytes` (a multiple of 16), run
+ * `iters` times on the context's stream: HBM bandwidth reference and the
+ * known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters.
+ * Returns the average ms per copy. */
+int sphe_debug_copy(sphe_ctx* ctx, size_t bytes, int iters, double* avg_ms);
 
 #ifdef __cplusplus
 }

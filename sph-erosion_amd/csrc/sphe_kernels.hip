@@ -1109,3 +1109,21 @@ void sphe_launch_mass(const DevState& D, int scene, double* part, double* out, h
     k_mass_partial<<<sphe_mass_blocks(), 256, 0, st>>>(D, scene, part);
     k_mass_final<<<1, 64, 0, st>>>(part, sphe_mass_blocks(), out);
 }
+
+// ------------------------------------------------------------ calibration
+
+namespace {
+// Plain streaming copy (16 bytes per thread per iteration): the reference
+// point for HBM bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE
+// counters of the step's kernels.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a, float4* __restrict__ b, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+}  // namespace
+
+void sphe_launch_copy(const void* a, void* b, int64_t n16, hipStream_t st)
+{
+    k_copy<<<2048, 256, 0, st>>>((const float4*)a, (float4*)b, n16);
+}

@@ -21,3 +21,4 @@ void sphe_launch_neighbours(const DevState& D, int scene, const int64_t* off, in
 void sphe_launch_keys(const DevState& D, int scene, int32_t* keys, hipStream_t st);
 int sphe_mass_blocks();
 void sphe_launch_mass(const DevState& D, int scene, double* part, double* out, hipStream_t st);
+void sphe_launch_copy(const void* a, void* b, int64_t n16, hipStream_t st);

@@ -1717,4 +1717,33 @@ int sphe_debug_density(sphe_ctx* c, int scene, double* rho)
     return SPHE_OK;
 }
 
+int sphe_debug_copy(sphe_ctx* c, size_t bytes, int iters, double* avg_ms)
+{
+    if (!c || bytes == 0 || bytes % 16 || iters < 1) return fail(c, SPHE_EARG, "bytes must be a positive multiple of 16");
+    void* a = nullptr;
+    void* b = nullptr;
+    HIPCHK(c, hipMalloc(&a, bytes));
+    hipError_t e = hipMalloc(&b, bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return fail(c, SPHE_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    hipEvent_t e0 = get_event(c), e1 = get_event(c);
+    e = hipMemsetAsync(a, 0, bytes, c->stream);
+    sphe_launch_copy(a, b, (int64_t)(bytes / 16), c->stream);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    for (int i = 0; i < iters && e == hipSuccess; ++i) sphe_launch_copy(a, b, (int64_t)(bytes / 16), c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    c->event_pool.push_back(e0);
+    c->event_pool.push_back(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(c, SPHE_EHIP, "copy: %s", hipGetErrorString(e));
+    if (avg_ms) *avg_ms = ms / iters;
+    return SPHE_OK;
+}
+
 }  // extern "C"

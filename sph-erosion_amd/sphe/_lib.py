@@ -99,6 +99,7 @@ SIGNATURES = {
     "sphe_debug_consts": (I, [P, I, P, I]),
     "sphe_debug_neighbours": (I, [P, I, P, P, P]),
     "sphe_debug_density": (I, [P, I, P]),
+    "sphe_debug_copy": (I, [P, C.c_size_t, I, C.POINTER(C.c_double)]),
 }
 
 _lib = None

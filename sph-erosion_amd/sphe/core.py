@@ -288,6 +288,11 @@ class Context:
                   "debug_neighbours")
         return counts, off, idx[:off[-1]]
 
+    def debug_copy(self, nbytes, iters=20):
+        ms = C.c_double()
+        self._chk(lib().sphe_debug_copy(self._p, nbytes, iters, C.byref(ms)), "debug_copy")
+        return ms.value
+
     def debug_density(self, scene):
         out = np.zeros(self.num_particles(scene))
         self._chk(lib().sphe_debug_density(self._p, scene, _ptr(out)), "debug_density")
