@@ -64,7 +64,7 @@ struct DevState {
     // state in unsorted slot order (between steps) and sorted order (in a step)
     void* u_pos; void* u_vel; int64_t* u_sed; int32_t* u_id;
     void* s_pos; void* s_vel; int64_t* s_sed; int32_t* s_id;
-    void* rho; void* aux;               // density; (p / rho^2, m / rho)
+    void* rho; void* posa;              // density; (x, y, z, p / rho^2) per sorted slot
     int32_t* key; uint32_t* arank;      // cell key and arrival rank of each unsorted slot
     int32_t* tmp_id; int32_t* tmp_key; int32_t* tmp_src;
     uint32_t* cnt; uint32_t* start; uint32_t* partial; int64_t npartial;

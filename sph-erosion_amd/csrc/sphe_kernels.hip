@@ -401,13 +401,16 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     uint32_t cnt = 0;
     const uint32_t cap = (uint32_t)D.maxn / G;
     uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn + l;
+    const uint32_t self = (uint32_t)slot;
     visit_strided<T, G>(S, K, D.start, pos, pi.x, pi.y, pi.z, kx, ky, kz, l,
                         [&](uint32_t j, T, T, T, T d2) {
                             T q = h2 - d2;
                             q = q > T(0) ? q : T(0);
                             sum.add((q * q) * q);
-                            if (cnt < cap) nbl[(int64_t)cnt * G] = j;
-                            ++cnt;
+                            if (j != self) {  // the force pass adds its own term first
+                                if (cnt < cap) nbl[(int64_t)cnt * G] = j;
+                                ++cnt;
+                            }
                         });
     D.nbc[slot * G + l] = cnt;
     if (cnt > cap) atomicMax(D.ovf, cnt * (uint32_t)G);
@@ -416,10 +419,12 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     const T rho = sum.value() * K.mc6;
     const T p = K.k * (rho - K.rho0);
     ((T*)D.rho)[slot] = rho;
-    V2<T> a;
-    a.x = p / (rho * rho);
-    a.y = K.mass / rho;
-    ((V2<T>*)D.aux)[slot] = a;
+    // (x, y, z, p / rho^2) and vel.w = m / rho: what the force pass gathers
+    // per neighbour, in two 16-byte loads
+    V4<T> pa = pi;
+    pa.w = p / (rho * rho);
+    ((V4<T>*)D.posa)[slot] = pa;
+    ((T*)D.s_vel)[4 * slot + 3] = K.mass / rho;
 }
 
 // ----------------------------------------------------------------- force
@@ -440,24 +445,28 @@ template <typename T, bool COMP>
 struct ForceAcc {
     Acc<T, COMP> fp0, fp1, fp2, fv0, fv1, fv2, nn0, nn1, nn2, lap;
 
-    __device__ __forceinline__ void pair(const SceneConsts<T>& K, bool self, T dx, T dy, T dz, T d2, T a_i, T a_j,
-                                         T b_j, const V4<T>& vi, const V4<T>& vj)
+    // The particle's own colour-Laplacian term (d = 0): added first, by lane 0.
+    __device__ __forceinline__ void self_term(const SceneConsts<T>& K, T b_i)
+    {
+        lap.add(b_i * (K.h2 * K.t3h2));
+    }
+
+    // One neighbour j != i.
+    __device__ __forceinline__ void pair(const SceneConsts<T>& K, T dx, T dy, T dz, T d2, T a_i, T a_j, T b_j,
+                                         const V4<T>& vi, const V4<T>& vj)
     {
         T q = K.h2 - d2;
         q = q > T(0) ? q : T(0);
         lap.add(b_j * (q * (K.t3h2 - T(7) * d2)));
-        if (self) return;
         const T r = d_sqrt(d2);
         const T hr = K.h - r;
-        T ux, uy, uz;
-        if (r < K.eps_r) {
-            // coincident pair: the reference points along normalize(1,1,1)
-            // (fluid_system.h:438-440)
-            ux = K.inv_sqrt3; uy = K.inv_sqrt3; uz = K.inv_sqrt3;
-        } else {
-            const T inv = T(1) / r;
-            ux = dx * inv; uy = dy * inv; uz = dz * inv;
-        }
+        // coincident pair: the reference points along normalize(1,1,1)
+        // (fluid_system.h:438-440); the quotient is discarded then
+        const bool near = r < K.eps_r;
+        const T inv = T(1) / r;
+        const T ux = near ? K.inv_sqrt3 : dx * inv;
+        const T uy = near ? K.inv_sqrt3 : dy * inv;
+        const T uz = near ? K.inv_sqrt3 : dz * inv;
         const T sp = (a_i + a_j) * (hr * hr);
         fp0.add(sp * ux);
         fp1.add(sp * uy);
@@ -508,20 +517,18 @@ __device__ __forceinline__ void force_share_walk(const DevState& D, const SceneD
                                                  ForceAcc<T, COMP>& acc, int32_t& nbmax, int32_t& count)
 {
     const V4<T>* __restrict__ pos = (const V4<T>*)D.s_pos;
+    const V4<T>* __restrict__ posa = (const V4<T>*)D.posa;
     const V4<T>* __restrict__ vel = (const V4<T>*)D.s_vel;
-    const V2<T>* __restrict__ aux = (const V2<T>*)D.aux;
     int kx, ky, kz;
     cell_of<T>(K, S.dims, pi.x, pi.y, pi.z, kx, ky, kz);
     visit_strided<T, G>(S, K, D.start, pos, pi.x, pi.y, pi.z, kx, ky, kz, l,
                         [&](uint32_t j, T dx, T dy, T dz, T d2) {
-                            const bool self = (int64_t)j == slot;
                             ++count;
-                            if (!self) {
-                                const int32_t idj = D.s_id[j];
-                                nbmax = idj > nbmax ? idj : nbmax;
-                            }
-                            const V2<T> aj = aux[j];
-                            acc.pair(K, self, dx, dy, dz, d2, a_i, aj.x, aj.y, vi, vel[j]);
+                            if ((int64_t)j == slot) return;  // own term: ForceAcc::self_term
+                            const int32_t idj = D.s_id[j];
+                            nbmax = idj > nbmax ? idj : nbmax;
+                            const V4<T> vj = vel[j];
+                            acc.pair(K, dx, dy, dz, d2, a_i, posa[j].w, vj.w, vi, vj);
                         });
 }
 
@@ -634,13 +641,13 @@ __device__ __forceinline__ int64_t erode_exchange(const SceneDev& S, const Scene
 }
 
 // Lane l's share of the force sums from the sub-list the density pass wrote
-// (its stencil positions l, l+G, ... that lie inside h, in order, self
-// included).  Only ~1 in 8 stencil candidates lies inside h, so walking the
-// stencil again would run the ~100-instruction pair body for the whole wave
-// whenever any lane accepts; the list holds accepted pairs only.  Entries are
-// read two ahead and neighbour data one ahead so the dependent gathers overlap
-// the pair arithmetic.  A lane whose share overflowed the list walks its
-// stencil positions instead (same pairs, same order, same result).
+// (its stencil positions l, l+G, ... that lie inside h, in order, the
+// particle itself left out).  Only ~1 in 8 stencil candidates lies inside h,
+// so walking the stencil again would run the ~100-instruction pair body for
+// the whole wave whenever any lane accepts; the list holds accepted pairs
+// only.  Two neighbours per iteration keep four gathers in flight.  A lane
+// whose share overflowed the list walks its stencil positions instead (same
+// pairs, same order, same result).
 template <typename T, bool COMP, int G>
 __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
                                            int64_t slot, int l, const V4<T>& pi, const V4<T>& vi, T a_i,
@@ -652,31 +659,36 @@ __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S
         force_share_walk<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc, nbmax, count);
         return;
     }
-    if (cnt == 0) return;
-    const V4<T>* __restrict__ pos = (const V4<T>*)D.s_pos;
+    const V4<T>* __restrict__ posa = (const V4<T>*)D.posa;
     const V4<T>* __restrict__ vel = (const V4<T>*)D.s_vel;
-    const V2<T>* __restrict__ aux = (const V2<T>*)D.aux;
     const uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn + l;
-    uint32_t jn = nbl[0];
-    uint32_t jn2 = cnt > 1 ? nbl[G] : jn;
-    V4<T> pn = pos[jn];
-    V4<T> vn = vel[jn];
-    V2<T> an = aux[jn];
-    for (uint32_t k = 0; k < cnt; ++k) {
-        const uint32_t j = jn;
-        const V4<T> pj = pn;
-        const V4<T> vj = vn;
-        const V2<T> aj = an;
-        jn = jn2;
-        if (k + 2 < cnt) jn2 = nbl[(k + 2) * G];
-        if (k + 1 < cnt) {
-            pn = pos[jn];
-            vn = vel[jn];
-            an = aux[jn];
+    uint32_t k = 0;
+    // two neighbours per iteration: four independent gathers in flight
+    for (; k + 1 < cnt; k += 2) {
+        const uint32_t ja = nbl[k * G];
+        const uint32_t jb = nbl[(k + 1) * G];
+        const V4<T> pa = posa[ja];
+        const V4<T> va = vel[ja];
+        const V4<T> pb = posa[jb];
+        const V4<T> vb = vel[jb];
+        {
+            const T dx = pi.x - pa.x, dy = pi.y - pa.y, dz = pi.z - pa.z;
+            const T d2 = (dx * dx + dy * dy) + dz * dz;
+            acc.pair(K, dx, dy, dz, d2, a_i, pa.w, va.w, vi, va);
         }
+        {
+            const T dx = pi.x - pb.x, dy = pi.y - pb.y, dz = pi.z - pb.z;
+            const T d2 = (dx * dx + dy * dy) + dz * dz;
+            acc.pair(K, dx, dy, dz, d2, a_i, pb.w, vb.w, vi, vb);
+        }
+    }
+    if (k < cnt) {
+        const uint32_t j = nbl[k * G];
+        const V4<T> pj = posa[j];
+        const V4<T> vj = vel[j];
         const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const T d2 = (dx * dx + dy * dy) + dz * dz;
-        acc.pair(K, (int64_t)j == slot, dx, dy, dz, d2, a_i, aj.x, aj.y, vi, vj);
+        acc.pair(K, dx, dy, dz, d2, a_i, pj.w, vj.w, vi, vj);
     }
 }
 
@@ -713,8 +725,9 @@ __global__ __launch_bounds__(512) void k_force(DevState D, int nblk, int remap)
     const V4<T> pi = spos[slot];
     const V4<T> vi = svel[slot];
     const T rho_i = ((const T*)D.rho)[slot];
-    const T a_i = ((const V2<T>*)D.aux)[slot].x;
+    const T a_i = ((const V4<T>*)D.posa)[slot].w;
     ForceAcc<T, COMP> acc;
+    if (l == 0) acc.self_term(K, vi.w);
     force_share<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc);
     acc.template merge<G>();
     if (l != 0) return;
@@ -832,8 +845,9 @@ __global__ void k_inspect(DevState D, int64_t slot, double* out)
     const V4<T> pi = ((const V4<T>*)D.s_pos)[slot];
     const V4<T> vi = ((const V4<T>*)D.s_vel)[slot];
     const T rho_i = ((const T*)D.rho)[slot];
-    const T a_i = ((const V2<T>*)D.aux)[slot].x;
+    const T a_i = ((const V4<T>*)D.posa)[slot].w;
     ForceAcc<T, COMP> acc;
+    if (l == 0) acc.self_term(K, vi.w);
     int32_t nbmax = -1, count = 0;
     force_share_walk<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc, nbmax, count);
     acc.template merge<G>();

@@ -364,7 +364,7 @@ int build_layout(sphe_ctx* c)
     DALLOC(D.u_id, Pn * 4);
     DALLOC(D.s_id, Pn * 4);
     DALLOC(D.rho, Pn * E);
-    DALLOC(D.aux, Pn * 2 * E);
+    DALLOC(D.posa, Pn * 4 * E);
     DALLOC(D.key, Pn * 4);
     DALLOC(D.arank, Pn * 4);
     DALLOC(D.tmp_id, Pn * 4);
