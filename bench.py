@@ -7,10 +7,14 @@ Workload (default): the single large scene, 262,144 particles in a column
 compensated sums.  A step is one pass of the whole hot path (sort, density,
 forces + integrate + collisions + erosion, terrain heights) over the scene.
 
-Prints one JSON line: particle-steps/s over the timed region (inputs resident
-in HBM), the HBM roofline of the dominant kernel from HIP events on the
-library's stream, the density error against the FP64 oracle, the plain FP64
-GPU path for comparison, and the reference step timed on the host's cores.
+Prints one JSON line: particle-steps/s over the timed region (K steps as
+the library runs them by default, one HIP graph per step, inputs resident in
+HBM); the same K steps replayed from a saved state kernel by kernel, with
+each kernel's start and end in HIP events on the library's stream, for the
+per-kernel times and the HBM roofline of the dominant kernel (the replay
+must end in a bit-identical state); the density error against the FP64
+oracle; the plain FP64 GPU path over the same steps; and the reference step
+timed on the host's cores.
 """
 import argparse
 import json
@@ -143,32 +147,38 @@ def main():
             log(f"trace: steps {c + 50:5d} ms/step {ms:.3f} neighbours mean {cnt.mean():.1f} max {cnt.max()}")
     ctx.step(a.warmup)
     ctx.sync()
+    ctx.save_state()
     mass0 = [ctx.mass(s) for s in sids]
 
-    # timed region: K steps, per-kernel HIP events on the library's stream
-    ctx.profile(True)
+    # timed region: K steps as the library runs them by default (each step
+    # one HIP graph launch on the library's stream), inputs resident in HBM
     ctx.sync()
     t0 = time.perf_counter()
     ctx.step(a.steps)
     ctx.sync()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
+    elapsed = time.perf_counter() - t0
+    status = ctx.status()
+    mass1 = [ctx.mass(s) for s in sids]
+    drift = max(abs((m1[0] + m1[1]) - (m0[0] + m0[1])) / (m0[0] + m0[1]) for m0, m1 in zip(mass0, mass1))
+    units_exact = all(m0[2] == m1[2] for m0, m1 in zip(mass0, mass1))
+    end_state = [ctx.state_f64(s) for s in sids]
+
+    # the same K steps again from the saved state, kernel by kernel: each
+    # kernel's start and end land in HIP events on the library's stream
+    ctx.restore_state()
+    ctx.profile(True)
+    ctx.sync()
+    tp = time.perf_counter()
+    ctx.step(a.steps)
+    ctx.sync()
+    prof_elapsed = time.perf_counter() - tp
     kernels = {}
     for name in ctx.profile_names():
         ms, cnt = ctx.profile_read(name)
         kernels[name] = dict(avg_ms=ms, launches=cnt, bytes=ctx.kernel_bytes(name) if name != "two_phase" else None)
     ctx.profile(False)
-    status = ctx.status()
-    mass1 = [ctx.mass(s) for s in sids]
-    drift = max(abs((m1[0] + m1[1]) - (m0[0] + m0[1])) / (m0[0] + m0[1]) for m0, m1 in zip(mass0, mass1))
-    units_exact = all(m0[2] == m1[2] for m0, m1 in zip(mass0, mass1))
-
-    # same steps as a HIP graph (the library's default mode)
-    ctx.sync()
-    tg = time.perf_counter()
-    ctx.step(a.steps)
-    ctx.sync()
-    graph_elapsed = time.perf_counter() - tg
+    replay_identical = all(np.array_equal(e[0], r[0]) and np.array_equal(e[1], r[1])
+                           for e, r in zip(end_state, [ctx.state_f64(s) for s in sids]))
 
     value = N * a.steps / elapsed
     dom = max((k for k in kernels if kernels[k]["bytes"]), key=lambda k: kernels[k]["avg_ms"])
@@ -190,7 +200,8 @@ def main():
                config=dict(workload=f"{a.workload}: {desc}", particles=N, scenes=len(scs),
                            precision="fp32 compensated (Kahan)", block=blk, xcd_remap=rm),
                roofline=roofline)
-    out["value_graph"] = N * a.steps / graph_elapsed
+    out["value_kernel_by_kernel"] = N * a.steps / prof_elapsed
+    out["replay_bit_identical"] = replay_identical
     out["kernels_ms"] = {k: round(v["avg_ms"], 5) for k, v in kernels.items()}
     out["mass_drift_rel"] = drift
     out["mass_units_exact"] = units_exact
@@ -219,11 +230,10 @@ def main():
         c64.set_tuning(blk, rm)
         c64.step(a.warmup)
         c64.sync()
-        k64 = max(1, a.steps // 5)
         t = time.perf_counter()
-        c64.step(k64)
+        c64.step(a.steps)
         c64.sync()
-        v64 = N * k64 / (time.perf_counter() - t)
+        v64 = N * a.steps / (time.perf_counter() - t)
         out["fp64_value"] = v64
         out["speedup_vs_fp64"] = value / v64
         c64.close()

@@ -234,7 +234,19 @@ int sphe_get_tuning(const sphe_ctx* ctx, int* block, int* xcd_remap);
  * for a given lane count (and differ between counts by summation order). */
 int sphe_set_lanes(sphe_ctx* ctx, int lanes);
 int sphe_get_lanes(const sphe_ctx* ctx, int* lanes);
+/* x slabs per grid cell (1, 2 or 4; default 4).  A cell's particles are
+ * ordered by (slab, id) and each stencil row is clipped to the slabs the
+ * smoothing sphere reaches, so the neighbour passes test fewer candidates.
+ * The slab count changes the canonical order, hence the summation order. */
+int sphe_set_subcells(sphe_ctx* ctx, int slabs);
+int sphe_get_subcells(const sphe_ctx* ctx, int* slabs);
 int sphe_autotune(sphe_ctx* ctx, int steps_per_trial);
+/* Saves the particle and terrain state of every scene (a host copy) and
+ * restores it later; stepping a restored state repeats the same steps bit
+ * for bit.  No counterpart in the reference (whose state is the particle
+ * vector itself); used to time and profile the same steps twice. */
+int sphe_save_state(sphe_ctx* ctx);
+int sphe_restore_state(sphe_ctx* ctx);
 
 /* ---- verification hooks (used by the parity tests) ---------------------- */
 

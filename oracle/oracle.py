@@ -48,7 +48,7 @@ class OrcScene(C.Structure):
                 ("tnx", C.c_int32), ("tnz", C.c_int32), ("t_org", C.c_double * 3),
                 ("t_cell", C.c_double), ("fx_exp", C.c_int32), ("erosion", C.c_int32),
                 ("kc", C.c_double), ("ke", C.c_double), ("kd", C.c_double), ("band", C.c_double),
-                ("lanes", C.c_int32)]
+                ("lanes", C.c_int32), ("slabs", C.c_int32)]
 
 
 class OrcConsts(C.Structure):
@@ -182,7 +182,7 @@ def make_scene(**kw):
     defaults = dict(h=0.0457, mass=0.02, rho0=998.29, k=3.0, visc=3.5, surf_tens=0.0728,
                     g=(0.0, -9.82, 0.0), dt=0.0, box_c=(0.0, 0.0, 0.0), box_e=0.2,
                     tnx=0, tnz=0, t_org=(0.0, 0.0, 0.0), t_cell=1.0, fx_exp=60, erosion=0,
-                    kc=0.05, ke=0.3, kd=0.3, band=0.0457, lanes=4)
+                    kc=0.05, ke=0.3, kd=0.3, band=0.0457, lanes=4, slabs=4)
     defaults.update(kw)
     for k, v in defaults.items():
         if isinstance(v, (tuple, list, np.ndarray)):

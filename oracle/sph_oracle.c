@@ -413,14 +413,17 @@ void orc_heights(const orc_consts* c, int tcells, const int64_t* vol_fx, float* 
 }
 
 /* ---- canonical order ---- */
-typedef struct { int32_t key; int32_t id; int32_t idx; } orc_sortrec;
+typedef struct { int32_t key; int32_t sub; int32_t id; int32_t idx; } orc_sortrec;
 
+/* canonical order: cell key, x slab, id, then input position */
 static int cmp_sortrec(const void* a, const void* b)
 {
     const orc_sortrec* x = (const orc_sortrec*)a;
     const orc_sortrec* y = (const orc_sortrec*)b;
     if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    if (x->sub != y->sub) return x->sub < y->sub ? -1 : 1;
     if (x->id != y->id) return x->id < y->id ? -1 : 1;
+    if (x->idx != y->idx) return x->idx < y->idx ? -1 : 1;
     return 0;
 }
 

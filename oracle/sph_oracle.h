@@ -101,6 +101,7 @@ typedef struct {
     int32_t erosion;                /* 0 = off, 1 = on */
     double kc, ke, kd, band;        /* capacity coefficient, erode/deposit rates, contact band */
     int32_t lanes;                  /* lanes per particle splitting the neighbour sums (1, 2, 4, 8) */
+    int32_t slabs;                  /* x slabs per grid cell (1, 2, 4): order and row clipping */
 } orc_scene;
 
 /* Derived per-scene constants, identical in meaning to the GPU's. */

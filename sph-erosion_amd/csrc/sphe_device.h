@@ -65,9 +65,13 @@ struct DevState {
     void* u_pos; void* u_vel; int64_t* u_sed; int32_t* u_id;
     void* s_pos; void* s_vel; int64_t* s_sed; int32_t* s_id;
     void* rho; void* posa;              // density; (x, y, z, p / rho^2) per sorted slot
-    int32_t* key; uint32_t* arank;      // cell key and arrival rank of each unsorted slot
+    int32_t* key; uint32_t* arank;      // cell key << sub_log2 | x slab, arrival rank in the cell
     int32_t* tmp_id; int32_t* tmp_key; int32_t* tmp_src;
     uint32_t* cnt; uint32_t* start; uint32_t* partial; int64_t npartial;
+    // cells are split into 2^sub_log2 slabs along x (sub_log2 <= 2); subw
+    // holds each cell's slab offsets (10 bits per inner boundary; bit 31:
+    // more than 1023 particles, no offsets)
+    uint32_t* subw; int32_t sub_log2;
     int64_t* vol; float* height;
     int64_t* dq; int32_t* tcell;        // two-phase erosion records
     uint32_t* tcnt; uint32_t* tstart; uint32_t* tpartial; int64_t ntpartial;

@@ -17,6 +17,7 @@
 // evaluated exactly as d2 = (dx*dx + dy*dy) + dz*dz <= T(h) with T(h) the
 // largest float whose square root is <= h.  Build with -ffp-contract=off.
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "sphe_device.h"
@@ -128,31 +129,103 @@ __device__ __forceinline__ void group_merge(A& a)
     for (int m = 1; m < G; m <<= 1) a.merge_xor(m);
 }
 
-// Stencil rows of one particle for lane l of its G-lane group: in every row
-// the lane takes positions l, l+G, l+2G, ... (two loads in flight) and calls
-// f(j, dx, dy, dz, d2) for those inside h.  Rows go dz outer, dy inner.
-template <typename T, int G, typename F>
-__device__ __forceinline__ void visit_strided(const SceneDev& S, const SceneConsts<T>& K,
-                                              const uint32_t* __restrict__ start,
-                                              const V4<T>* __restrict__ pos, T xi, T yi, T zi,
-                                              int kx, int ky, int kz, int l, F&& f)
+// x sub-cell of a position inside its cell: the cell is split into
+// 2^sub_log2 slabs along x, and a cell's particles are sorted by (slab, id).
+// gx * 2^s is exact, so the slab index lies in [0, 2^s) for every position
+// (clamped and non-finite ones included).
+template <typename T>
+__device__ __forceinline__ int sub_cell(T gx, int kx, int nx, int sl)
+{
+    return cidx<T>(d_floor(gx * T(1 << sl)), (nx << sl) - 1) - (kx << sl);
+}
+
+// Lower bound, in cell units, of the distance from a coordinate at fraction
+// f of its cell to the next cell, less a margin for rounding.
+#define SPHE_ROW_MARGIN 0.015625
+template <typename T>
+__device__ __forceinline__ T row_gap(T f)
+{
+    const T a = f - T(SPHE_ROW_MARGIN);
+    return a > T(0) ? a : T(0);
+}
+
+// The stencil rows of a particle, clipped to what the smoothing sphere can
+// reach: calls row(j0, j1) with the slot range of each row (dz outer, dy
+// inner) that may hold a neighbour.  In cell units the radius is
+// h / cell < 1; a row whose y and z gaps g satisfy gy^2 + gz^2 > 1 is
+// skipped, and the others keep the x sub-cells that meet
+// [gx - w, gx + w], w = sqrt(1 - gy^2 - gz^2) + margin.  Every neighbour is
+// inside the range: the bounds are monotone in the position and include a
+// margin far above the rounding of the cell coordinates.  Cells with more
+// than 1023 particles carry no slab offsets and are taken whole.
+template <typename T, typename F>
+__device__ __forceinline__ void for_each_row(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
+                                             T xi, T yi, T zi, F&& row)
 {
     const int nx = S.dims[0], ny = S.dims[1], nz = S.dims[2];
-    const int x0 = kx > 0 ? kx - 1 : 0;
-    const int x1 = kx < nx - 1 ? kx + 1 : nx - 1;
+    const T gx = (xi - K.org[0]) * K.inv_cell;
+    const T gy = (yi - K.org[1]) * K.inv_cell;
+    const T gz = (zi - K.org[2]) * K.inv_cell;
+    const int kx = cidx<T>(d_floor(gx), nx - 1);
+    const int ky = cidx<T>(d_floor(gy), ny - 1);
+    const int kz = cidx<T>(d_floor(gz), nz - 1);
+    const T fy = gy - T(ky), fz = gz - T(kz);
+    const T ylo = row_gap<T>(fy), yhi = row_gap<T>(T(1) - fy);
+    const T zlo = row_gap<T>(fz), zhi = row_gap<T>(T(1) - fz);
+    const int sl = D.sub_log2, smax = (1 << sl) - 1;
+    const T scale = T(1 << sl);
+    const int fmax = (nx << sl) - 1;
+    const int flo_min = (kx > 0 ? kx - 1 : 0) << sl;
+    const int fhi_max = ((kx < nx - 1 ? kx + 1 : nx - 1) << sl) + smax;
     const uint32_t lo = (uint32_t)S.p_begin, hi = (uint32_t)(S.p_begin + S.n);
-    const T thr = K.thr;
+    const uint32_t* __restrict__ start = D.start;
+    const uint32_t* __restrict__ subw = D.subw;
     for (int dzc = -1; dzc <= 1; ++dzc) {
         const int z = kz + dzc;
         if (z < 0 || z >= nz) continue;
+        const T mz = dzc < 0 ? zlo : (dzc > 0 ? zhi : T(0));
         for (int dyc = -1; dyc <= 1; ++dyc) {
             const int y = ky + dyc;
             if (y < 0 || y >= ny) continue;
+            const T my = dyc < 0 ? ylo : (dyc > 0 ? yhi : T(0));
+            const T r2 = my * my + mz * mz;
+            if (r2 > T(1)) continue;
+            const T w = d_sqrt(T(1) - r2) + T(SPHE_ROW_MARGIN);
+            int flo = cidx<T>(d_floor((gx - w) * scale), fmax);
+            int fhi = cidx<T>(d_floor((gx + w) * scale), fmax);
+            flo = flo > flo_min ? flo : flo_min;
+            fhi = fhi < fhi_max ? fhi : fhi_max;
             const int64_t rk = (int64_t)S.cell_off + ((int64_t)z * ny + y) * nx;
+            const int clo = flo >> sl, chi = fhi >> sl;
+            uint32_t j0 = start[rk + clo];
+            uint32_t j1 = start[rk + chi + 1];
+            if (sl) {
+                const int slo = flo & smax, shi = fhi & smax;
+                if (slo) {
+                    const uint32_t w0 = subw[rk + clo];
+                    if (!(w0 >> 31)) j0 += (w0 >> (10 * slo - 10)) & 1023u;
+                }
+                if (shi != smax) {
+                    const uint32_t w1 = subw[rk + chi];
+                    if (!(w1 >> 31)) j1 = start[rk + chi] + ((w1 >> (10 * shi)) & 1023u);
+                }
+            }
             // rows of real cells lie inside [p_begin, p_begin + n); the clamp
             // only matters if the counts were inconsistent
-            const uint32_t j0 = max(start[rk + x0], lo);
-            const uint32_t j1 = min(start[rk + x1 + 1], hi);
+            row(max(j0, lo), min(j1, hi));
+        }
+    }
+}
+
+// Stencil rows of one particle for lane l of its G-lane group: in every row
+// the lane takes positions l, l+G, l+2G, ... (two loads in flight) and calls
+// f(j, dx, dy, dz, d2) for those inside h.
+template <typename T, int G, typename F>
+__device__ __forceinline__ void visit_strided(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
+                                              const V4<T>* __restrict__ pos, T xi, T yi, T zi, int l, F&& f)
+{
+    const T thr = K.thr;
+    for_each_row<T>(D, S, K, xi, yi, zi, [&](uint32_t j0, uint32_t j1) {
             uint32_t j = j0 + (uint32_t)l;
             for (; j + G < j1; j += 2 * G) {
                 const V4<T> pa = pos[j];
@@ -170,34 +243,16 @@ __device__ __forceinline__ void visit_strided(const SceneDev& S, const SceneCons
                 const T d2 = (dx * dx + dy * dy) + dz * dz;
                 if (d2 <= thr) f(j, dx, dy, dz, d2);
             }
-        }
-    }
+    });
 }
 
-// Visits the candidates of the 27-cell stencil of (kx, ky, kz) in ascending
-// slot order (rows dz, dy; x-cells contiguous inside a row) and calls
-// f(j, dx, dy, dz, d2) for those inside the smoothing radius.
+// Visits the candidates of the clipped stencil rows in ascending slot order
+// and calls f(j, dx, dy, dz, d2) for those inside the smoothing radius.
 template <typename T, typename F>
-__device__ __forceinline__ void visit_neighbours(const SceneDev& S, const SceneConsts<T>& K,
-                                                 const uint32_t* __restrict__ start,
-                                                 const V4<T>* __restrict__ pos,
-                                                 T xi, T yi, T zi, int kx, int ky, int kz, F&& f)
+__device__ __forceinline__ void visit_neighbours(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
+                                                 const V4<T>* __restrict__ pos, T xi, T yi, T zi, F&& f)
 {
-    const int nx = S.dims[0], ny = S.dims[1], nz = S.dims[2];
-    const int x0 = kx > 0 ? kx - 1 : 0;
-    const int x1 = kx < nx - 1 ? kx + 1 : nx - 1;
-    const uint32_t lo = (uint32_t)S.p_begin, hi = (uint32_t)(S.p_begin + S.n);
-    for (int dzc = -1; dzc <= 1; ++dzc) {
-        const int z = kz + dzc;
-        if (z < 0 || z >= nz) continue;
-        for (int dyc = -1; dyc <= 1; ++dyc) {
-            const int y = ky + dyc;
-            if (y < 0 || y >= ny) continue;
-            const int64_t rk = (int64_t)S.cell_off + ((int64_t)z * ny + y) * nx;
-            // rows of real cells lie inside [p_begin, p_begin + n); the clamp
-            // only matters if the counts were inconsistent
-            const uint32_t j0 = max(start[rk + x0], lo);
-            const uint32_t j1 = min(start[rk + x1 + 1], hi);
+    for_each_row<T>(D, S, K, xi, yi, zi, [&](uint32_t j0, uint32_t j1) {
             uint32_t j = j0;
             for (; j + 1 < j1; j += 2) {  // two loads in flight
                 const V4<T> pa = pos[j];
@@ -215,8 +270,30 @@ __device__ __forceinline__ void visit_neighbours(const SceneDev& S, const SceneC
                 const T d2 = (dx * dx + dy * dy) + dz * dz;
                 if (d2 <= K.thr) f(j, dx, dy, dz, d2);
             }
-        }
-    }
+    });
+}
+
+// Counting-sort key of a position: the cell key shifted left by sub_log2,
+// or-ed with the x slab; the cell's count is taken with an atomic.
+template <typename T>
+__device__ __forceinline__ void count_position(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
+                                               int64_t slot, T x, T y, T z)
+{
+    const T gx = (x - K.org[0]) * K.inv_cell;
+    const int kx = cidx<T>(d_floor(gx), S.dims[0] - 1);
+    const int ky = cidx<T>(d_floor((y - K.org[1]) * K.inv_cell), S.dims[1] - 1);
+    const int kz = cidx<T>(d_floor((z - K.org[2]) * K.inv_cell), S.dims[2] - 1);
+    const int64_t key = S.cell_off + ((int64_t)kz * S.dims[1] + ky) * S.dims[0] + kx;
+    const int sub = sub_cell<T>(gx, kx, S.dims[0], D.sub_log2);
+    D.key[slot] = (int32_t)((key << D.sub_log2) | sub);
+    D.arank[slot] = atomicAdd(&D.cnt[key], 1u);
+}
+
+__device__ __forceinline__ void count_pad(const DevState& D, const SceneDev& S, int64_t slot)
+{
+    const int64_t key = S.cell_off + S.ncell;
+    D.key[slot] = (int32_t)(key << D.sub_log2);
+    D.arank[slot] = atomicAdd(&D.cnt[key], 1u);
 }
 
 // ------------------------------------------------------------------ sort
@@ -227,18 +304,13 @@ __global__ __launch_bounds__(256) void k_hash_count(DevState D)
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= D.P) return;
     const SceneDev& S = D.scenes[wave_scene(D.chunk_scene, slot)];
-    int64_t key;
     if (slot < S.p_begin + S.n) {
         const SceneConsts<T>& K = SceneSel<T>::get(S);
         const V4<T> p = ((const V4<T>*)D.u_pos)[slot];
-        int kx, ky, kz;
-        cell_of<T>(K, S.dims, p.x, p.y, p.z, kx, ky, kz);
-        key = S.cell_off + ((int64_t)kz * S.dims[1] + ky) * S.dims[0] + kx;
+        count_position<T>(D, S, K, slot, p.x, p.y, p.z);
     } else {
-        key = S.cell_off + S.ncell;  // pad cell
+        count_pad(D, S, slot);
     }
-    D.key[slot] = (int32_t)key;
-    D.arank[slot] = atomicAdd(&D.cnt[key], 1u);
 }
 
 #define SCAN_ITEMS 16
@@ -268,15 +340,47 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
     return base + x - v;
 }
 
+// SCAN_ITEMS consecutive words per thread, as 16-byte accesses where the
+// whole run is inside [0, n) (tiles start at multiples of 64 bytes).
+__device__ __forceinline__ void load_run(const uint32_t* __restrict__ p, int64_t base, int64_t n,
+                                         uint32_t* v)
+{
+    if (base + SCAN_ITEMS <= n) {
+        const uint4* q = (const uint4*)(p + base);
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS / 4; ++k) {
+            const uint4 a = q[k];
+            v[4 * k] = a.x; v[4 * k + 1] = a.y; v[4 * k + 2] = a.z; v[4 * k + 3] = a.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i) v[i] = base + i < n ? p[base + i] : 0u;
+    }
+}
+
+__device__ __forceinline__ void store_run(uint32_t* __restrict__ p, int64_t base, int64_t n, const uint32_t* v)
+{
+    if (base + SCAN_ITEMS <= n) {
+        uint4* q = (uint4*)(p + base);
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS / 4; ++k) q[k] = make_uint4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i)
+            if (base + i < n) p[base + i] = v[i];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
                                                      uint32_t* __restrict__ partial)
 {
     __shared__ uint32_t lds[4];
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    load_run(in, base, n, v);
     uint32_t s = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i)
-        if (base + i < n) s += in[base + i];
+    for (int i = 0; i < SCAN_ITEMS; ++i) s += v[i];
     uint32_t tot;
     block_exclusive_scan(s, &tot, lds);
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
@@ -297,41 +401,44 @@ __global__ __launch_bounds__(256) void k_scan_partials(uint32_t* __restrict__ pa
 }
 
 // start[i] = exclusive prefix of counts; counts are reset to zero for the
-// next step's arrivals.
+// next step's arrivals, and so is the optional per-cell word array clear.
 __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, uint32_t* __restrict__ start,
-                                                   int64_t n, const uint32_t* __restrict__ partial)
+                                                   int64_t n, const uint32_t* __restrict__ partial,
+                                                   uint32_t* __restrict__ clear)
 {
     __shared__ uint32_t lds[4];
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
+    load_run(cnt, base, n, v);
     uint32_t s = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i) {
-        v[i] = base + i < n ? cnt[base + i] : 0u;
-        s += v[i];
-    }
+    for (int i = 0; i < SCAN_ITEMS; ++i) s += v[i];
     uint32_t tot;
     uint32_t run = block_exclusive_scan(s, &tot, lds) + partial[blockIdx.x];
+    uint32_t z[SCAN_ITEMS];
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
-        if (base + i < n) {
-            start[base + i] = run;
-            cnt[base + i] = 0u;
-        }
-        run += v[i];
+        const uint32_t c = v[i];
+        v[i] = run;
+        run += c;
+        z[i] = 0u;
     }
+    store_run(start, base, n, v);
+    store_run(cnt, base, n, z);
+    if (clear) store_run(clear, base, n, z);
 }
 
 __global__ __launch_bounds__(256) void k_scatter(DevState D)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= D.P) return;
-    const int32_t key = D.key[p];
-    if (key < 0 || (int64_t)key >= D.C) {
+    const int32_t key = D.key[p];  // cell key << sub_log2 | slab
+    const int32_t c = key >> D.sub_log2;
+    if (key < 0 || (int64_t)c >= D.C) {
         atomicOr(D.status, 2u);
         return;
     }
-    const uint32_t d = D.start[key] + D.arank[p];
+    const uint32_t d = D.start[c] + D.arank[p];
     if ((int64_t)d >= D.P) {  // counts inconsistent with the slots: report, never write out of range
         atomicOr(D.status, 2u);
         return;
@@ -341,15 +448,20 @@ __global__ __launch_bounds__(256) void k_scatter(DevState D)
     D.tmp_src[d] = (int32_t)p;
 }
 
-// Canonical order inside a cell: ascending id (ties, which only duplicate
-// ids produce, by previous slot).
+// Canonical order inside a cell: ascending x slab, then ascending id (ties,
+// which only duplicate ids produce, by previous slot).  The particle ranked
+// first also writes the cell's slab offsets: for s = 1 .. 2^sub_log2 - 1 the
+// number of the cell's particles in slabs below s, 10 bits each from bit
+// 10 (s - 1); a cell of more than 1023 particles gets bit 31 instead.
 template <typename T>
 __global__ __launch_bounds__(256) void k_reorder(DevState D)
 {
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= D.P) return;
-    const int32_t c = D.tmp_key[d];
-    if (c < 0 || (int64_t)c >= D.C) {
+    const int sl = D.sub_log2;
+    const int32_t fk = D.tmp_key[d];
+    const int32_t c = fk >> sl;
+    if (fk < 0 || (int64_t)c >= D.C) {
         atomicOr(D.status, 2u);
         return;
     }
@@ -357,17 +469,24 @@ __global__ __launch_bounds__(256) void k_reorder(DevState D)
     const uint32_t s1 = (int64_t)D.start[c + 1] <= D.P ? D.start[c + 1] : (uint32_t)D.P;
     const int32_t my = D.tmp_id[d];
     const int32_t src = D.tmp_src[d];
-    uint32_t rank = 0;
+    // one count field per slab boundary: a particle in slab k adds 1 to the
+    // fields of the boundaries s > k
+    const uint32_t fields = sl == 2 ? 0x100401u : (sl == 1 ? 1u : 0u);
+    const uint32_t smask = (1u << sl) - 1u;
+    uint32_t rank = 0, below = 0;
     for (uint32_t j = s0; j < s1; ++j) {
+        const int32_t ok = D.tmp_key[j];
         const int32_t o = D.tmp_id[j];
-        rank += (o < my) ? 1u : 0u;
-        if (o == my && D.tmp_src[j] < src) rank += 1u;
+        const bool before = ok < fk || (ok == fk && (o < my || (o == my && D.tmp_src[j] < src)));
+        rank += before ? 1u : 0u;
+        below += fields & (~0u << (10u * ((uint32_t)ok & smask)));
     }
     const uint32_t f = s0 + rank;
     if ((int64_t)f >= D.P || src < 0 || (int64_t)src >= D.P) {
         atomicOr(D.status, 2u);
         return;
     }
+    if (rank == 0) D.subw[c] = s1 - s0 > 1023u ? 0x80000000u : below;
     ((V4<T>*)D.s_pos)[f] = ((const V4<T>*)D.u_pos)[src];
     ((V4<T>*)D.s_vel)[f] = ((const V4<T>*)D.u_vel)[src];
     D.s_sed[f] = D.u_sed[src];
@@ -394,24 +513,22 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     const SceneConsts<T>& K = SceneSel<T>::get(S);
     const V4<T>* __restrict__ pos = (const V4<T>*)D.s_pos;
     const V4<T> pi = pos[slot];
-    int kx, ky, kz;
-    cell_of<T>(K, S.dims, pi.x, pi.y, pi.z, kx, ky, kz);
     Acc<T, COMP> sum;
     const T h2 = K.h2;
     uint32_t cnt = 0;
     const uint32_t cap = (uint32_t)D.maxn / G;
     uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn + l;
     const uint32_t self = (uint32_t)slot;
-    visit_strided<T, G>(S, K, D.start, pos, pi.x, pi.y, pi.z, kx, ky, kz, l,
-                        [&](uint32_t j, T, T, T, T d2) {
-                            T q = h2 - d2;
-                            q = q > T(0) ? q : T(0);
-                            sum.add((q * q) * q);
-                            if (j != self) {  // the force pass adds its own term first
-                                if (cnt < cap) nbl[(int64_t)cnt * G] = j;
-                                ++cnt;
-                            }
-                        });
+    visit_strided<T, G>(D, S, K, pos, pi.x, pi.y, pi.z, l, [&](uint32_t j, T, T, T, T d2) {
+        T q = h2 - d2;
+        q = q > T(0) ? q : T(0);
+        sum.add((q * q) * q);
+        // the list leaves out the particle itself: the force pass adds its
+        // own term first
+        const bool app = j != self;
+        if (app & (cnt < cap)) nbl[(int64_t)cnt * G] = j;
+        cnt += app ? 1u : 0u;
+    });
     D.nbc[slot * G + l] = cnt;
     if (cnt > cap) atomicMax(D.ovf, cnt * (uint32_t)G);
     group_merge<G>(sum);
@@ -436,10 +553,49 @@ struct ForceOut {
     int32_t count;
 };
 
-// Pressure, viscosity, colour normal and colour Laplacian sums of one
-// particle (fluid_system.h:136-150, 165-172) with the constants factored out
-// of the sums.  pair() takes one neighbour (self included) in canonical
-// order; finish() applies the constants and forms the acceleration
+// The terms one neighbour j != i adds to the ten force sums: pressure,
+// viscosity, colour normal and colour Laplacian (fluid_system.h:136-150,
+// 165-172) with the constants factored out.
+template <typename T> struct Terms {
+    T lap, fp0, fp1, fp2, fv0, fv1, fv2, nn0, nn1, nn2;
+};
+
+template <typename T>
+__device__ __forceinline__ Terms<T> pair_terms(const SceneConsts<T>& K, T dx, T dy, T dz, T d2, T a_i, T a_j,
+                                               T b_j, const V4<T>& vi, const V4<T>& vj)
+{
+    Terms<T> t;
+    T q = K.h2 - d2;
+    q = q > T(0) ? q : T(0);
+    t.lap = b_j * (q * (K.t3h2 - T(7) * d2));
+    const T r = d_sqrt(d2);
+    const T hr = K.h - r;
+    // coincident pair: the reference points along normalize(1,1,1)
+    // (fluid_system.h:438-440); the quotient is discarded then
+    const bool near = r < K.eps_r;
+    const T inv = T(1) / r;
+    const T ux = near ? K.inv_sqrt3 : dx * inv;
+    const T uy = near ? K.inv_sqrt3 : dy * inv;
+    const T uz = near ? K.inv_sqrt3 : dz * inv;
+    const T sp = (a_i + a_j) * (hr * hr);
+    t.fp0 = sp * ux;
+    t.fp1 = sp * uy;
+    t.fp2 = sp * uz;
+    const T wv = b_j * hr;
+    t.fv0 = (vj.x - vi.x) * wv;
+    t.fv1 = (vj.y - vi.y) * wv;
+    t.fv2 = (vj.z - vi.z) * wv;
+    const T wn = b_j * (q * q);
+    t.nn0 = dx * wn;
+    t.nn1 = dy * wn;
+    t.nn2 = dz * wn;
+    return t;
+}
+
+// Force sums of one lane.  A lane's neighbours enter in consecutive pairs:
+// the two terms are added in plain arithmetic (first + second) and the pair
+// sum enters the compensated sum; an odd last neighbour enters alone.
+// finish() applies the constants and forms the acceleration
 // (fluid_system.h:151-154, 163, 177, 318-321).
 template <typename T, bool COMP>
 struct ForceAcc {
@@ -451,34 +607,20 @@ struct ForceAcc {
         lap.add(b_i * (K.h2 * K.t3h2));
     }
 
-    // One neighbour j != i.
-    __device__ __forceinline__ void pair(const SceneConsts<T>& K, T dx, T dy, T dz, T d2, T a_i, T a_j, T b_j,
-                                         const V4<T>& vi, const V4<T>& vj)
+    __device__ __forceinline__ void add(const Terms<T>& t)
     {
-        T q = K.h2 - d2;
-        q = q > T(0) ? q : T(0);
-        lap.add(b_j * (q * (K.t3h2 - T(7) * d2)));
-        const T r = d_sqrt(d2);
-        const T hr = K.h - r;
-        // coincident pair: the reference points along normalize(1,1,1)
-        // (fluid_system.h:438-440); the quotient is discarded then
-        const bool near = r < K.eps_r;
-        const T inv = T(1) / r;
-        const T ux = near ? K.inv_sqrt3 : dx * inv;
-        const T uy = near ? K.inv_sqrt3 : dy * inv;
-        const T uz = near ? K.inv_sqrt3 : dz * inv;
-        const T sp = (a_i + a_j) * (hr * hr);
-        fp0.add(sp * ux);
-        fp1.add(sp * uy);
-        fp2.add(sp * uz);
-        const T wv = b_j * hr;
-        fv0.add((vj.x - vi.x) * wv);
-        fv1.add((vj.y - vi.y) * wv);
-        fv2.add((vj.z - vi.z) * wv);
-        const T wn = b_j * (q * q);
-        nn0.add(dx * wn);
-        nn1.add(dy * wn);
-        nn2.add(dz * wn);
+        lap.add(t.lap);
+        fp0.add(t.fp0); fp1.add(t.fp1); fp2.add(t.fp2);
+        fv0.add(t.fv0); fv1.add(t.fv1); fv2.add(t.fv2);
+        nn0.add(t.nn0); nn1.add(t.nn1); nn2.add(t.nn2);
+    }
+
+    __device__ __forceinline__ void add2(const Terms<T>& a, const Terms<T>& b)
+    {
+        lap.add(a.lap + b.lap);
+        fp0.add(a.fp0 + b.fp0); fp1.add(a.fp1 + b.fp1); fp2.add(a.fp2 + b.fp2);
+        fv0.add(a.fv0 + b.fv0); fv1.add(a.fv1 + b.fv1); fv2.add(a.fv2 + b.fv2);
+        nn0.add(a.nn0 + b.nn0); nn1.add(a.nn1 + b.nn1); nn2.add(a.nn2 + b.nn2);
     }
 
     template <int G>
@@ -519,17 +661,21 @@ __device__ __forceinline__ void force_share_walk(const DevState& D, const SceneD
     const V4<T>* __restrict__ pos = (const V4<T>*)D.s_pos;
     const V4<T>* __restrict__ posa = (const V4<T>*)D.posa;
     const V4<T>* __restrict__ vel = (const V4<T>*)D.s_vel;
-    int kx, ky, kz;
-    cell_of<T>(K, S.dims, pi.x, pi.y, pi.z, kx, ky, kz);
-    visit_strided<T, G>(S, K, D.start, pos, pi.x, pi.y, pi.z, kx, ky, kz, l,
+    Terms<T> pend;
+    bool has = false;  // pend holds the first of a pair
+    visit_strided<T, G>(D, S, K, pos, pi.x, pi.y, pi.z, l,
                         [&](uint32_t j, T dx, T dy, T dz, T d2) {
                             ++count;
                             if ((int64_t)j == slot) return;  // own term: ForceAcc::self_term
                             const int32_t idj = D.s_id[j];
                             nbmax = idj > nbmax ? idj : nbmax;
                             const V4<T> vj = vel[j];
-                            acc.pair(K, dx, dy, dz, d2, a_i, posa[j].w, vj.w, vi, vj);
+                            const Terms<T> t = pair_terms<T>(K, dx, dy, dz, d2, a_i, posa[j].w, vj.w, vi, vj);
+                            if (has) acc.add2(pend, t);
+                            else pend = t;
+                            has = !has;
                         });
+    if (has) acc.add(pend);
 }
 
 // Dominant-axis box response (fluid_system.h:342-347 with collisionS
@@ -671,16 +817,12 @@ __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S
         const V4<T> va = vel[ja];
         const V4<T> pb = posa[jb];
         const V4<T> vb = vel[jb];
-        {
-            const T dx = pi.x - pa.x, dy = pi.y - pa.y, dz = pi.z - pa.z;
-            const T d2 = (dx * dx + dy * dy) + dz * dz;
-            acc.pair(K, dx, dy, dz, d2, a_i, pa.w, va.w, vi, va);
-        }
-        {
-            const T dx = pi.x - pb.x, dy = pi.y - pb.y, dz = pi.z - pb.z;
-            const T d2 = (dx * dx + dy * dy) + dz * dz;
-            acc.pair(K, dx, dy, dz, d2, a_i, pb.w, vb.w, vi, vb);
-        }
+        const T ax = pi.x - pa.x, ay = pi.y - pa.y, az = pi.z - pa.z;
+        const T a2 = (ax * ax + ay * ay) + az * az;
+        const T bx = pi.x - pb.x, by = pi.y - pb.y, bz = pi.z - pb.z;
+        const T b2 = (bx * bx + by * by) + bz * bz;
+        acc.add2(pair_terms<T>(K, ax, ay, az, a2, a_i, pa.w, va.w, vi, va),
+                 pair_terms<T>(K, bx, by, bz, b2, a_i, pb.w, vb.w, vi, vb));
     }
     if (k < cnt) {
         const uint32_t j = nbl[k * G];
@@ -688,7 +830,7 @@ __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S
         const V4<T> vj = vel[j];
         const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const T d2 = (dx * dx + dy * dy) + dz * dz;
-        acc.pair(K, dx, dy, dz, d2, a_i, pj.w, vj.w, vi, vj);
+        acc.add(pair_terms<T>(K, dx, dy, dz, d2, a_i, pj.w, vj.w, vi, vj));
     }
 }
 
@@ -696,7 +838,7 @@ __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S
 // (xor shuffles), and lane 0 integrates, collides, erodes, stores the new
 // state and counts the particle's next cell.
 template <typename T, bool COMP, int G>
-__global__ __launch_bounds__(512) void k_force(DevState D, int nblk, int remap)
+__device__ __forceinline__ void force_body(const DevState& D, int nblk, int remap)
 {
     const int b = xcd_block(blockIdx.x, nblk, remap);
     const int64_t gt = (int64_t)b * blockDim.x + threadIdx.x;
@@ -715,9 +857,7 @@ __global__ __launch_bounds__(512) void k_force(DevState D, int nblk, int remap)
         uvel[slot] = svel[slot];
         D.u_sed[slot] = D.s_sed[slot];
         D.u_id[slot] = D.s_id[slot];
-        const int64_t key = S.cell_off + S.ncell;
-        D.key[slot] = (int32_t)key;
-        D.arank[slot] = atomicAdd(&D.cnt[key], 1u);
+        count_pad(D, S, slot);
         if (S.erosion == 2) D.tcell[slot] = -1;
         return;
     }
@@ -774,11 +914,23 @@ __global__ __launch_bounds__(512) void k_force(DevState D, int nblk, int remap)
           d_finite(v[2])))
         atomicOr(D.status, 1u);
     // cell count of the new position for the next step's sort
-    int kx, ky, kz;
-    cell_of<T>(K, S.dims, p[0], p[1], p[2], kx, ky, kz);
-    const int64_t key = S.cell_off + ((int64_t)kz * S.dims[1] + ky) * S.dims[0] + kx;
-    D.key[slot] = (int32_t)key;
-    D.arank[slot] = atomicAdd(&D.cnt[key], 1u);
+    count_position<T>(D, S, K, slot, p[0], p[1], p[2]);
+}
+
+// FP32: at most 80 registers (6 waves per SIMD): the list loop fits, and
+// only the rarely taken stencil walk of overflowing lanes spills.  Unbounded,
+// the walk raises the count to 101 and the occupancy of the whole kernel to 4.
+template <bool COMP, int G>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_force_f32(DevState D, int nblk,
+                                                                                              int remap)
+{
+    force_body<float, COMP, G>(D, nblk, remap);
+}
+
+template <int G>
+__global__ __launch_bounds__(512) void k_force_f64(DevState D, int nblk, int remap)
+{
+    force_body<double, false, G>(D, nblk, remap);
 }
 
 // --------------------------------------------------------------- terrain
@@ -888,12 +1040,10 @@ __global__ __launch_bounds__(256) void k_neighbours(DevState D, int scene, const
     const SceneConsts<T>& K = SceneSel<T>::get(S);
     const V4<T>* pos = (const V4<T>*)D.s_pos;
     const V4<T> pi = pos[slot];
-    int kx, ky, kz;
-    cell_of<T>(K, S.dims, pi.x, pi.y, pi.z, kx, ky, kz);
     const int64_t local = slot - S.p_begin;
     int32_t c = 0;
     const int64_t o = idx ? off[local] : 0;
-    visit_neighbours<T>(S, K, D.start, pos, pi.x, pi.y, pi.z, kx, ky, kz,
+    visit_neighbours<T>(D, S, K, pos, pi.x, pi.y, pi.z,
                         [&](uint32_t j, T, T, T, T) {
                             if (idx) idx[o + c] = D.s_id[j];
                             ++c;
@@ -993,91 +1143,105 @@ inline unsigned grid_of(int64_t n, int block) { return (unsigned)((n + block - 1
         else { CALL_F64; }                                      \
     } while (0)
 
-void sphe_launch_hash(const DevState& D, hipStream_t st)
+// Launches kern.  With profiling events, through hipExtLaunchKernel: e0 and
+// e1 then take the kernel's own start and end times, with no marker packets
+// between kernels.
+template <typename... P, typename... A>
+static void klaunch(void (*kern)(P...), dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                    A... args)
 {
-    if (D.P == 0) return;
-    if (D.precision == 2) k_hash_count<double><<<grid_of(D.P, 256), 256, 0, st>>>(D);
-    else k_hash_count<float><<<grid_of(D.P, 256), 256, 0, st>>>(D);
+    if (e0 || e1) hipExtLaunchKernelGGL(kern, grid, block, 0, st, e0, e1, 0, args...);
+    else hipLaunchKernelGGL(kern, grid, block, 0, st, args...);
 }
 
-void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, hipStream_t st)
+void sphe_launch_hash(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+{
+    if (D.P == 0) return;
+    if (D.precision == 2) klaunch(k_hash_count<double>, grid_of(D.P, 256), 256, st, e0, e1, D);
+    else klaunch(k_hash_count<float>, grid_of(D.P, 256), 256, st, e0, e1, D);
+}
+
+void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, uint32_t* clear,
+                      hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    k_scan_reduce<<<(unsigned)nb, 256, 0, st>>>(cnt, n, partial);
+    klaunch(k_scan_reduce, (unsigned)nb, 256, st, e0, nullptr, (const uint32_t*)cnt, n, partial);
     k_scan_partials<<<1, 256, 0, st>>>(partial, nb);
-    k_scan_down<<<(unsigned)nb, 256, 0, st>>>(cnt, start, n, partial);
+    klaunch(k_scan_down, (unsigned)nb, 256, st, nullptr, e1, cnt, start, n, (const uint32_t*)partial, clear);
 }
 
 int64_t sphe_scan_partials(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
-void sphe_launch_scatter(const DevState& D, hipStream_t st)
+void sphe_launch_scatter(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     if (D.P == 0) return;
-    k_scatter<<<grid_of(D.P, 256), 256, 0, st>>>(D);
+    klaunch(k_scatter, grid_of(D.P, 256), 256, st, e0, e1, D);
 }
 
-void sphe_launch_reorder(const DevState& D, hipStream_t st)
+void sphe_launch_reorder(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     if (D.P == 0) return;
-    if (D.precision == 2) k_reorder<double><<<grid_of(D.P, 256), 256, 0, st>>>(D);
-    else k_reorder<float><<<grid_of(D.P, 256), 256, 0, st>>>(D);
+    if (D.precision == 2) klaunch(k_reorder<double>, grid_of(D.P, 256), 256, st, e0, e1, D);
+    else klaunch(k_reorder<float>, grid_of(D.P, 256), 256, st, e0, e1, D);
 }
 
 template <int G>
-static void launch_density_g(const DevState& D, int block, int remap, hipStream_t st)
+static void launch_density_g(const DevState& D, int block, int remap, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     const int nblk = (int)grid_of(D.P * G, block);
     SPHE_PREC_SWITCH(D.precision,
-                     (k_density<float, true, G><<<nblk, block, 0, st>>>(D, nblk, remap)),
-                     (k_density<float, false, G><<<nblk, block, 0, st>>>(D, nblk, remap)),
-                     (k_density<double, false, G><<<nblk, block, 0, st>>>(D, nblk, remap)));
+                     (klaunch(k_density<float, true, G>, nblk, block, st, e0, e1, D, nblk, remap)),
+                     (klaunch(k_density<float, false, G>, nblk, block, st, e0, e1, D, nblk, remap)),
+                     (klaunch(k_density<double, false, G>, nblk, block, st, e0, e1, D, nblk, remap)));
 }
 
 template <int G>
-static void launch_force_g(const DevState& D, int block, int remap, hipStream_t st)
+static void launch_force_g(const DevState& D, int block, int remap, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     const int nblk = (int)grid_of(D.P * G, block);
     SPHE_PREC_SWITCH(D.precision,
-                     (k_force<float, true, G><<<nblk, block, 0, st>>>(D, nblk, remap)),
-                     (k_force<float, false, G><<<nblk, block, 0, st>>>(D, nblk, remap)),
-                     (k_force<double, false, G><<<nblk, block, 0, st>>>(D, nblk, remap)));
+                     (klaunch(k_force_f32<true, G>, nblk, block, st, e0, e1, D, nblk, remap)),
+                     (klaunch(k_force_f32<false, G>, nblk, block, st, e0, e1, D, nblk, remap)),
+                     (klaunch(k_force_f64<G>, nblk, block, st, e0, e1, D, nblk, remap)));
 }
 
-void sphe_launch_density(const DevState& D, int block, int remap, int lanes, hipStream_t st)
+void sphe_launch_density(const DevState& D, int block, int remap, int lanes, hipStream_t st, hipEvent_t e0,
+                         hipEvent_t e1)
 {
     if (D.P == 0) return;
     switch (lanes) {
-    case 1: launch_density_g<1>(D, block, remap, st); break;
-    case 2: launch_density_g<2>(D, block, remap, st); break;
-    case 8: launch_density_g<8>(D, block, remap, st); break;
-    default: launch_density_g<4>(D, block, remap, st); break;
+    case 1: launch_density_g<1>(D, block, remap, st, e0, e1); break;
+    case 2: launch_density_g<2>(D, block, remap, st, e0, e1); break;
+    case 8: launch_density_g<8>(D, block, remap, st, e0, e1); break;
+    default: launch_density_g<4>(D, block, remap, st, e0, e1); break;
     }
 }
 
-void sphe_launch_force(const DevState& D, int block, int remap, int lanes, hipStream_t st)
+void sphe_launch_force(const DevState& D, int block, int remap, int lanes, hipStream_t st, hipEvent_t e0,
+                       hipEvent_t e1)
 {
     if (D.P == 0) return;
     switch (lanes) {
-    case 1: launch_force_g<1>(D, block, remap, st); break;
-    case 2: launch_force_g<2>(D, block, remap, st); break;
-    case 8: launch_force_g<8>(D, block, remap, st); break;
-    default: launch_force_g<4>(D, block, remap, st); break;
+    case 1: launch_force_g<1>(D, block, remap, st, e0, e1); break;
+    case 2: launch_force_g<2>(D, block, remap, st, e0, e1); break;
+    case 8: launch_force_g<8>(D, block, remap, st, e0, e1); break;
+    default: launch_force_g<4>(D, block, remap, st, e0, e1); break;
     }
 }
 
-void sphe_launch_heights(const DevState& D, hipStream_t st)
+void sphe_launch_heights(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     if (D.TC == 0) return;
-    k_heights<<<grid_of(D.TC, 256), 256, 0, st>>>(D);
+    klaunch(k_heights, grid_of(D.TC, 256), 256, st, e0, e1, D);
 }
 
-void sphe_launch_two_phase(const DevState& D, hipStream_t st)
+void sphe_launch_two_phase(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     if (D.TC == 0 || D.P == 0) return;
-    k_tcount<<<grid_of(D.P, 256), 256, 0, st>>>(D);
-    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, st);
+    klaunch(k_tcount, grid_of(D.P, 256), 256, st, e0, nullptr, D);
+    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, nullptr, st);
     k_tscatter<<<grid_of(D.P, 256), 256, 0, st>>>(D);
-    k_tapply<<<grid_of(D.TC, 256), 256, 0, st>>>(D);
+    klaunch(k_tapply, grid_of(D.TC, 256), 256, st, nullptr, e1, D);
 }
 
 void sphe_launch_clear_status(const DevState& D, hipStream_t st) { k_clear_status<<<1, 64, 0, st>>>(D.status); }

@@ -5,15 +5,20 @@
 
 #include "sphe_device.h"
 
-void sphe_launch_hash(const DevState& D, hipStream_t st);
-void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, hipStream_t st);
+// e0 / e1 (optional): profiling events that take the start of the first and
+// the end of the last kernel a launcher issues.
+void sphe_launch_hash(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, uint32_t* clear,
+                      hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 int64_t sphe_scan_partials(int64_t n);
-void sphe_launch_scatter(const DevState& D, hipStream_t st);
-void sphe_launch_reorder(const DevState& D, hipStream_t st);
-void sphe_launch_density(const DevState& D, int block, int remap, int lanes, hipStream_t st);
-void sphe_launch_force(const DevState& D, int block, int remap, int lanes, hipStream_t st);
-void sphe_launch_heights(const DevState& D, hipStream_t st);
-void sphe_launch_two_phase(const DevState& D, hipStream_t st);
+void sphe_launch_scatter(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+void sphe_launch_reorder(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+void sphe_launch_density(const DevState& D, int block, int remap, int lanes, hipStream_t st,
+                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+void sphe_launch_force(const DevState& D, int block, int remap, int lanes, hipStream_t st,
+                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+void sphe_launch_heights(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+void sphe_launch_two_phase(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void sphe_launch_clear_status(const DevState& D, hipStream_t st);
 void sphe_launch_inspect(const DevState& D, int64_t slot, int lanes, double* out, hipStream_t st);
 void sphe_launch_neighbours(const DevState& D, int scene, const int64_t* off, int32_t* counts, int32_t* idx,

@@ -90,10 +90,22 @@ struct sphe_ctx {
     int remap = 1;
     int maxn = 128;  // neighbour-list capacity per particle (grown when a step overflows it)
     int lanes = 4;   // lanes per particle in the density and force passes
+    int sub_log2 = 2;  // x slabs per cell: 2^sub_log2
 
     double* d_mass_part = nullptr;
     double* d_mass_out = nullptr;
     double* d_inspect = nullptr;
+
+    // sphe_save_state / sphe_restore_state: particle and terrain state of
+    // every scene, in the host staging layout
+    struct Saved {
+        std::vector<double> pos, vel;
+        std::vector<int32_t> ids;
+        std::vector<int64_t> sed, vol;
+        int next_id;
+    };
+    std::vector<Saved> saved;
+    bool has_saved = false;
 };
 
 namespace {
@@ -372,6 +384,9 @@ int build_layout(sphe_ctx* c)
     DALLOC(D.tmp_src, Pn * 4);
     DALLOC(D.cnt, (size_t)(C + 1) * 4);
     DALLOC(D.start, (size_t)(C + 1) * 4);
+    DALLOC(D.subw, (size_t)(C + 1) * 4);
+    D.sub_log2 = c->sub_log2;
+    while (D.sub_log2 > 0 && (C << D.sub_log2) > INT32_MAX) --D.sub_log2;  // keys are int32
     D.npartial = sphe_scan_partials(C + 1);
     DALLOC(D.partial, (size_t)D.npartial * 4);
     DALLOC(D.vol, (size_t)TC * 8);
@@ -410,6 +425,7 @@ int build_layout(sphe_ctx* c)
     HIPCHK(c, hipMemcpyAsync(cs, hcs.data(), hcs.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(tcs, htcs.data(), htcs.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(D.cnt, 0, (size_t)(C + 1) * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(D.subw, 0, (size_t)(C + 1) * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(D.tcnt, 0, (size_t)(TC + 1) * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(D.status, 0, 64, c->stream));
     HIPCHK(c, hipMemsetAsync(D.ovf, 0, 64, c->stream));
@@ -613,24 +629,23 @@ hipEvent_t get_event(sphe_ctx* c)
     return e;
 }
 
+// Profiling: a pair of events the launcher hands to its first and last
+// kernel (hipExtLaunchKernel), so they carry the kernels' own start and end
+// times.  Without profiling both are null and launches are plain.
 struct Timed {
     sphe_ctx* c;
     const char* name;
-    hipEvent_t a = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
     Timed(sphe_ctx* ctx, const char* n) : c(ctx), name(n)
     {
         if (c->profiling) {
             a = get_event(c);
-            (void)hipEventRecord(a, c->stream);
+            b = get_event(c);
         }
     }
     ~Timed()
     {
-        if (c->profiling) {
-            hipEvent_t b = get_event(c);
-            (void)hipEventRecord(b, c->stream);
-            c->pending.push_back(ProfRec{name, a, b});
-        }
+        if (c->profiling) c->pending.push_back(ProfRec{name, a, b});
     }
 };
 
@@ -646,15 +661,15 @@ void launch_sort(sphe_ctx* c)
     const DevState& D = c->D;
     {
         Timed t(c, "scan");
-        sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, c->stream);
+        sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, D.subw, c->stream, t.a, t.b);
     }
     {
         Timed t(c, "scatter");
-        sphe_launch_scatter(D, c->stream);
+        sphe_launch_scatter(D, c->stream, t.a, t.b);
     }
     {
         Timed t(c, "reorder");
-        sphe_launch_reorder(D, c->stream);
+        sphe_launch_reorder(D, c->stream, t.a, t.b);
     }
 }
 
@@ -664,29 +679,29 @@ void launch_step(sphe_ctx* c)
     launch_sort(c);
     {
         Timed t(c, "density");
-        sphe_launch_density(D, c->block, c->remap, c->lanes, c->stream);
+        sphe_launch_density(D, c->block, c->remap, c->lanes, c->stream, t.a, t.b);
     }
     {
         Timed t(c, "force");
-        sphe_launch_force(D, c->block, c->remap, c->lanes, c->stream);
+        sphe_launch_force(D, c->block, c->remap, c->lanes, c->stream, t.a, t.b);
     }
     if (any_erosion(c, SPHE_EROSION_TWO_PHASE)) {
         Timed t(c, "two_phase");
-        sphe_launch_two_phase(D, c->stream);
+        sphe_launch_two_phase(D, c->stream, t.a, t.b);
     }
     if (any_erosion(c, -1)) {
         Timed t(c, "heights");
-        sphe_launch_heights(D, c->stream);
+        sphe_launch_heights(D, c->stream, t.a, t.b);
     }
 }
 
 std::string plan_signature(const sphe_ctx* c)
 {
     char b[256];
-    snprintf(b, sizeof(b), "%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%d", c->precision, c->block, c->remap,
+    snprintf(b, sizeof(b), "%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%d/%d", c->precision, c->block, c->remap,
              c->lanes, (long long)c->D.P, (long long)c->D.C, (long long)c->D.TC, (int)any_erosion(c, -1),
              (int)any_erosion(c, SPHE_EROSION_TWO_PHASE), (const void*)c->D.u_pos, (const void*)c->D.nbl,
-             c->D.maxn);
+             c->D.maxn, c->D.sub_log2);
     return b;
 }
 
@@ -715,7 +730,7 @@ int recount(sphe_ctx* c)
     HIPCHK(c, hipMemsetAsync(c->D.cnt, 0, (size_t)(c->D.C + 1) * 4, c->stream));
     {
         Timed t(c, "hash");
-        sphe_launch_hash(c->D, c->stream);
+        sphe_launch_hash(c->D, c->stream, t.a, t.b);
     }
     HIPCHK(c, hipGetLastError());
     c->counted = true;
@@ -765,17 +780,28 @@ int do_steps(sphe_ctx* c, int nsteps)
     }
     HIPCHK(c, hipGetLastError());
     const bool graph = c->use_graph && !c->profiling;
-    if (graph) {
-        r = ensure_graph(c);
-        if (r) return r;
+    // A long call checks the list overflow flag every kList steps, so a
+    // state that compresses during the call does not keep walking its
+    // stencil to the end of the call.
+    const int kList = 64;
+    for (int k0 = 0; k0 < nsteps; k0 += kList) {
+        if (k0 > 0) {
+            r = grow_lists(c);
+            if (r) return r;
+        }
+        if (graph) {
+            r = ensure_graph(c);
+            if (r) return r;
+        }
+        const int k1 = nsteps - k0 < kList ? nsteps : k0 + kList;
+        for (int k = k0; k < k1; ++k) {
+            if (graph) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+            else launch_step(c);
+        }
+        HIPCHK(c, hipGetLastError());
+        c->stepped = true;
     }
-    for (int k = 0; k < nsteps; ++k) {
-        if (graph) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
-        else launch_step(c);
-    }
-    HIPCHK(c, hipGetLastError());
     c->host_valid = false;
-    c->stepped = true;
     return SPHE_OK;
 }
 
@@ -1543,11 +1569,69 @@ int sphe_get_lanes(const sphe_ctx* c, int* lanes)
     return SPHE_OK;
 }
 
+int sphe_set_subcells(sphe_ctx* c, int slabs)
+{
+    if (!c) return SPHE_EARG;
+    if (slabs != 1 && slabs != 2 && slabs != 4) return fail(c, SPHE_EARG, "slabs must be 1, 2 or 4");
+    int sl = slabs == 4 ? 2 : (slabs == 2 ? 1 : 0);
+    c->sub_log2 = sl;
+    if (c->layout_valid) {
+        while (sl > 0 && (c->D.C << sl) > INT32_MAX) --sl;
+        if (sl != c->D.sub_log2) {
+            c->D.sub_log2 = sl;
+            c->counted = false;  // the keys of the counted state carry the slab
+        }
+    }
+    return SPHE_OK;
+}
+
+int sphe_get_subcells(const sphe_ctx* c, int* slabs)
+{
+    if (!c || !slabs) return SPHE_EARG;
+    *slabs = 1 << (c->layout_valid ? c->D.sub_log2 : c->sub_log2);
+    return SPHE_OK;
+}
+
 int sphe_get_tuning(const sphe_ctx* c, int* block, int* remap)
 {
     if (!c) return SPHE_EARG;
     if (block) *block = c->block;
     if (remap) *remap = c->remap;
+    return SPHE_OK;
+}
+
+int sphe_save_state(sphe_ctx* c)
+{
+    if (!c) return SPHE_EARG;
+    int r = sync_to_host(c);
+    if (r) return r;
+    c->saved.clear();
+    for (const Scene& s : c->scenes) c->saved.push_back({s.pos, s.vel, s.ids, s.sed, s.vol, s.next_id});
+    c->has_saved = true;
+    return SPHE_OK;
+}
+
+int sphe_restore_state(sphe_ctx* c)
+{
+    if (!c) return SPHE_EARG;
+    if (!c->has_saved) return fail(c, SPHE_ESTATE, "no saved state");
+    if (c->saved.size() != c->scenes.size()) return fail(c, SPHE_ESTATE, "scenes were added after the state was saved");
+    bool same = true;
+    for (size_t i = 0; i < c->scenes.size(); ++i)
+        same = same && c->scenes[i].ids.size() == c->saved[i].ids.size() &&
+               c->scenes[i].vol.size() == c->saved[i].vol.size();
+    if (!same) {
+        int r = begin_host_edit(c);  // particle or terrain sizes changed: new layout
+        if (r) return r;
+    }
+    for (size_t i = 0; i < c->scenes.size(); ++i) {
+        Scene& s = c->scenes[i];
+        const auto& v = c->saved[i];
+        s.pos = v.pos; s.vel = v.vel; s.ids = v.ids; s.sed = v.sed; s.vol = v.vol; s.next_id = v.next_id;
+    }
+    c->host_valid = true;
+    c->device_valid = false;  // uploaded (and counted) on the next use
+    c->stepped = false;
     return SPHE_OK;
 }
 

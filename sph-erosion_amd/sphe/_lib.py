@@ -95,6 +95,10 @@ SIGNATURES = {
     "sphe_autotune": (I, [P, I]),
     "sphe_set_lanes": (I, [P, I]),
     "sphe_get_lanes": (I, [P, C.POINTER(I)]),
+    "sphe_save_state": (I, [P]),
+    "sphe_restore_state": (I, [P]),
+    "sphe_set_subcells": (I, [P, I]),
+    "sphe_get_subcells": (I, [P, C.POINTER(I)]),
     "sphe_debug_cell_keys": (I, [P, I, P]),
     "sphe_debug_consts": (I, [P, I, P, I]),
     "sphe_debug_neighbours": (I, [P, I, P, P, P]),

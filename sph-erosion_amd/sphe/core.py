@@ -258,6 +258,20 @@ class Context:
         self._chk(lib().sphe_get_lanes(self._p, C.byref(v)), "get_lanes")
         return v.value
 
+    def save_state(self):
+        self._chk(lib().sphe_save_state(self._p), "save_state")
+
+    def restore_state(self):
+        self._chk(lib().sphe_restore_state(self._p), "restore_state")
+
+    def set_subcells(self, slabs):
+        self._chk(lib().sphe_set_subcells(self._p, slabs), "set_subcells")
+
+    def get_subcells(self):
+        v = C.c_int()
+        self._chk(lib().sphe_get_subcells(self._p, C.byref(v)), "get_subcells")
+        return v.value
+
     def autotune(self, steps_per_trial=5):
         self._chk(lib().sphe_autotune(self._p, steps_per_trial), "autotune")
         return self.get_tuning()

@@ -205,6 +205,22 @@ def test_batch_equals_single_scenes():
         assert np.array_equal(batch.terrain_volume_fx(sid), single.terrain_volume_fx(ss))
 
 
+def test_save_restore_repeats_the_same_steps():
+    """A restored state steps to the same bits, also after the lists grew."""
+    sc = small_dam()
+    ctx, sid = gpu_context(sc)
+    ctx.step(5)
+    ctx.save_state()
+    ctx.step(20)
+    p1, v1 = ctx.state_f64(sid)
+    s1, vol1 = ctx.sediment_fx(sid), ctx.terrain_volume_fx(sid)
+    ctx.restore_state()
+    ctx.step(20)
+    p2, v2 = ctx.state_f64(sid)
+    assert np.array_equal(p1, p2) and np.array_equal(v1, v2)
+    assert np.array_equal(s1, ctx.sediment_fx(sid)) and np.array_equal(vol1, ctx.terrain_volume_fx(sid))
+
+
 def test_autotune_restores_state_and_counts():
     """Autotune steps re-uploaded copies of the state; the cell counters are
     cleared before every recount, and the tuned result equals an untuned run
@@ -252,6 +268,57 @@ def test_lane_split_bit_exact_twin(lanes):
         st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], sc.ids, st["vol"], st["height"], real=4, comp=1)
     assert np.array_equal(gp, st["pos"]) and np.array_equal(gv, st["vel"])
     assert np.array_equal(ctx.sediment_fx(sid), st["sed"])
+
+
+@pytest.mark.parametrize("slabs", [1, 2])
+def test_slab_split_bit_exact_twin(slabs):
+    """The x-slab count sets the canonical order and the clipped rows; the
+    twin with the same count reproduces the GPU steps bit for bit."""
+    sc = small_dam()
+    ctx, sid = gpu_context(sc)
+    ctx.set_subcells(slabs)
+    assert ctx.get_subcells() == slabs
+    pos, vel, sed, vol, height, fx = oracle_inputs(ctx, sid, sc)
+    osc = orc_scene_from(sc.params, sc.terrain, fx)
+    osc.slabs = slabs
+    ctx.step(2)
+    gp, gv = ctx.state_f64(sid)
+    st = dict(pos=pos, vel=vel, sed=sed, vol=vol, height=height)
+    for _ in range(2):
+        st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], sc.ids, st["vol"], st["height"], real=4, comp=1)
+    assert np.array_equal(gp, st["pos"]) and np.array_equal(gv, st["vel"])
+    assert np.array_equal(ctx.sediment_fx(sid), st["sed"])
+
+
+def test_crowded_cell_taken_whole():
+    """A cell of 2300 particles (more than the 1023 the slab offsets can
+    count) with probes along x through it: neighbour sets equal brute force
+    and two steps equal the twin bit for bit."""
+    rng = np.random.default_rng(21)
+    h = np.float32(0.0457)
+    cell = float(h) * (1 + 2.0 ** -8)
+    org = -0.2
+    fr = np.concatenate([rng.uniform(0.02, 0.22, 1300), rng.uniform(0.78, 0.98, 1000)])
+    blob = org + cell * np.stack([2 + fr, 2 + rng.uniform(0.3, 0.7, 2300), 2 + rng.uniform(0.3, 0.7, 2300)], 1)
+    probes = org + cell * np.stack([rng.uniform(0.3, 4.7, 400), np.full(400, 2.5), np.full(400, 2.5)], 1)
+    pos = np.concatenate([blob, probes, rng.uniform(-0.19, 0.19, size=(1500, 3))]).astype(np.float32)
+    n = len(pos)
+    p = sphe.default_params(dt=1e-5)
+    ctx = sphe.Context()
+    sid = ctx.add_scene(p)
+    ctx.set_particles(sid, pos)
+    cnt, off, idx = O.neighbours_brute(pos, O.pred_threshold(h))
+    gcnt, goff, gidx = ctx.debug_neighbours(sid)
+    assert np.array_equal(gcnt, cnt)
+    for i in range(0, n, 11):
+        assert np.array_equal(np.sort(gidx[goff[i]:goff[i + 1]]), np.sort(idx[off[i]:off[i + 1]]))
+    osc = orc_scene_from(p)
+    ctx.step(2)
+    st = dict(pos=pos.astype(np.float64), vel=np.zeros((n, 3)), sed=np.zeros(n, np.int64))
+    for _ in range(2):
+        st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], np.arange(n, dtype=np.int32), real=4, comp=1)
+    gp, gv = ctx.state_f64(sid)
+    assert np.array_equal(gp, st["pos"]) and np.array_equal(gv, st["vel"])
 
 
 def test_list_overflow_walks_the_same_pairs():

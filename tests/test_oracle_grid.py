@@ -35,6 +35,59 @@ def test_grid_neighbours_equal_brute_force():
     assert np.array_equal(out["nbmax"], nbmax)
 
 
+def _clip_cases():
+    """Inputs that stress the clipped stencil rows: a dense blob of more than
+    1023 particles in one cell (no slab offsets), lattices whose points sit
+    on cell and slab boundaries and at exactly h apart, and particles outside
+    the grid (clamped into the edge cells)."""
+    rng = np.random.default_rng(21)
+    h = np.float32(0.0457)
+    cell = float(h) * (1 + 2.0 ** -8)
+    org = -0.2
+    # cell (2, 2, 2) holds 2300 particles in its first and last x slab, and a
+    # line of probes along x through it ends its rows at every slab
+    fr = np.concatenate([rng.uniform(0.02, 0.22, 1300), rng.uniform(0.78, 0.98, 1000)])
+    blob = org + cell * np.stack([2 + fr, 2 + rng.uniform(0.3, 0.7, 2300), 2 + rng.uniform(0.3, 0.7, 2300)], 1)
+    probes = org + cell * np.stack([rng.uniform(0.3, 4.7, 400), np.full(400, 2.5), np.full(400, 2.5)], 1)
+    blob = np.concatenate([blob, probes]).astype(np.float32)
+    bg = rng.uniform(-0.19, 0.19, size=(1500, 3)).astype(np.float32)
+    g = np.arange(-3, 4)
+    lat_h = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) * h  # spacing h
+    lat_c = (org + np.stack(np.meshgrid(np.arange(12), np.arange(6), np.arange(6), indexing="ij"), -1)
+             .reshape(-1, 3) * (cell / 4.0)).astype(np.float32)  # on slab and cell edges
+    out = rng.uniform(-0.3, 0.3, size=(300, 3)).astype(np.float32)  # partly outside the box
+    return {"blob": np.concatenate([blob, bg]), "lattice_h": lat_h.astype(np.float32),
+            "edges": lat_c, "outside": np.concatenate([out, bg[:500]])}
+
+
+@pytest.mark.parametrize("slabs", [1, 2, 4])
+@pytest.mark.parametrize("case", ["blob", "lattice_h", "edges", "outside"])
+@pytest.mark.parametrize("real", [4, 8])
+def test_clipped_rows_keep_every_neighbour(slabs, case, real):
+    """The clipped stencil rows (x slabs, skipped rows, cells taken whole
+    above 1023 particles) still reach every neighbour the all-pairs FP32
+    predicate finds: per-particle counts and largest neighbour ids equal
+    brute force."""
+    pos = _clip_cases()[case]
+    n = len(pos)
+    sc = O.make_scene(dt=0.0, slabs=slabs)
+    out = O.orc_step(sc, pos, np.zeros((n, 3)), np.zeros(n, np.int64), np.arange(n, dtype=np.int32),
+                     real=real, comp=1)
+    thr = O.pred_threshold(np.float32(0.0457)) if real == 4 else float(np.float32(0.0457)) ** 2
+    if real == 4:
+        cnt, off, idx = O.neighbours_brute(pos, thr)
+    else:
+        d = pos.astype(np.float64)
+        dx, dy, dz = (d[:, None, a] - d[None, :, a] for a in range(3))
+        m = ((dx * dx + dy * dy) + dz * dz) <= thr
+        cnt = m.sum(1)
+        idx = None
+    assert np.array_equal(out["nbcount"], cnt)
+    if idx is not None:
+        nbmax = np.array([max([j for j in idx[off[i]:off[i + 1]] if j != i], default=-1) for i in range(n)])
+        assert np.array_equal(out["nbmax"], nbmax)
+
+
 def test_cell_keys_cover_neighbours():
     """Every neighbour pair lies in adjacent cells (the 27-cell stencil)."""
     rng = np.random.default_rng(5)
