@@ -79,12 +79,12 @@ struct DevState {
     uint32_t* status;
     double* red;                        // reduction scratch
     int64_t nred;
-    // neighbour sub-lists written by the density pass for the force pass:
-    // lane l of a particle's G-lane group owns nbl[slot * maxn + k * G + l]
-    // for k < min(nbc[slot * G + l], maxn / G); a lane whose count exceeds
-    // maxn / G walks its stencil positions again
+    // neighbour lists written by the density pass for the force pass: the
+    // neighbours of sorted slot i (self excluded) in canonical order at
+    // nbl[i * maxn + m], m < min(nbc[i], maxn); a particle whose count
+    // exceeds maxn walks its stencil again in the force pass
     uint32_t* nbl;
     uint32_t* nbc;
     int32_t maxn;
-    uint32_t* ovf;  // largest neighbour count (x lanes) that overflowed a sub-list
+    uint32_t* ovf;  // largest neighbour count that overflowed a list
 };

@@ -61,6 +61,14 @@ __device__ __forceinline__ int wave_scene(const int32_t* chunk_scene, int64_t sl
     return __builtin_amdgcn_readfirstlane(chunk_scene[slot >> 6]);
 }
 
+// Element j of a per-slot array, addressed by a 32-bit byte offset from the
+// (wave-uniform) base: the layout keeps every such array under 4 GiB.
+template <typename V>
+__device__ __forceinline__ V at(const V* __restrict__ base, uint32_t j)
+{
+    return *(const V*)((const char*)base + j * (uint32_t)sizeof(V));
+}
+
 template <typename T>
 __device__ __forceinline__ int cidx(T f, int nmax)
 {
@@ -215,35 +223,6 @@ __device__ __forceinline__ void for_each_row(const DevState& D, const SceneDev& 
             row(max(j0, lo), min(j1, hi));
         }
     }
-}
-
-// Stencil rows of one particle for lane l of its G-lane group: in every row
-// the lane takes positions l, l+G, l+2G, ... (two loads in flight) and calls
-// f(j, dx, dy, dz, d2) for those inside h.
-template <typename T, int G, typename F>
-__device__ __forceinline__ void visit_strided(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
-                                              const V4<T>* __restrict__ pos, T xi, T yi, T zi, int l, F&& f)
-{
-    const T thr = K.thr;
-    for_each_row<T>(D, S, K, xi, yi, zi, [&](uint32_t j0, uint32_t j1) {
-            uint32_t j = j0 + (uint32_t)l;
-            for (; j + G < j1; j += 2 * G) {
-                const V4<T> pa = pos[j];
-                const V4<T> pb = pos[j + G];
-                const T ax = xi - pa.x, ay = yi - pa.y, az = zi - pa.z;
-                const T a2 = (ax * ax + ay * ay) + az * az;
-                if (a2 <= thr) f(j, ax, ay, az, a2);
-                const T bx = xi - pb.x, by = yi - pb.y, bz = zi - pb.z;
-                const T b2 = (bx * bx + by * by) + bz * bz;
-                if (b2 <= thr) f(j + G, bx, by, bz, b2);
-            }
-            if (j < j1) {
-                const V4<T> pj = pos[j];
-                const T dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
-                const T d2 = (dx * dx + dy * dy) + dz * dz;
-                if (d2 <= thr) f(j, dx, dy, dz, d2);
-            }
-    });
 }
 
 // Visits the candidates of the clipped stencil rows in ascending slot order
@@ -495,11 +474,14 @@ __global__ __launch_bounds__(256) void k_reorder(DevState D)
 
 // --------------------------------------------------------------- density
 
-// One G-lane group per particle (lane l = thread index mod G).  Each lane
-// sums its share of the stencil and writes its accepted neighbours to its own
-// sub-list (nbl[slot * maxn + k * G + l], k < maxn / G; its count in
-// nbc[slot * G + l]) for the force pass; the group then merges the partial
-// sums and lane 0 writes rho and (p / rho^2, m / rho).
+// One G-lane group per particle (lane l = thread index mod G).  In every
+// clipped stencil row lane l takes the candidates at positions l, l+G, ...
+// (two per iteration) for its share of the density sum.  The group writes
+// one neighbour list per particle (self left out) in canonical order,
+// nbl[slot * maxn + m], m < maxn, count in nbc[slot]: a ballot over the
+// group orders each iteration's accepted candidates (the first of the pair
+// in lane order, then the second).  The group then merges the partial sums
+// and lane 0 writes rho and (p / rho^2, m / rho).
 template <typename T, bool COMP, int G>
 __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap)
 {
@@ -514,25 +496,60 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     const V4<T>* __restrict__ pos = (const V4<T>*)D.s_pos;
     const V4<T> pi = pos[slot];
     Acc<T, COMP> sum;
-    const T h2 = K.h2;
-    uint32_t cnt = 0;
-    const uint32_t cap = (uint32_t)D.maxn / G;
-    uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn + l;
+    const T h2 = K.h2, thr = K.thr;
+    const uint32_t cap = (uint32_t)D.maxn;
+    uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn;
     const uint32_t self = (uint32_t)slot;
-    visit_strided<T, G>(D, S, K, pos, pi.x, pi.y, pi.z, l, [&](uint32_t j, T, T, T, T d2) {
-        T q = h2 - d2;
-        q = q > T(0) ? q : T(0);
-        sum.add((q * q) * q);
-        // the list leaves out the particle itself: the force pass adds its
-        // own term first
-        const bool app = j != self;
-        if (app & (cnt < cap)) nbl[(int64_t)cnt * G] = j;
-        cnt += app ? 1u : 0u;
+    const int gfirst = (int)(threadIdx.x & (SPHE_WAVE - 1)) & ~(G - 1);  // the group's first lane in the wave
+    const uint32_t gmask = (1u << G) - 1u, below = (1u << l) - 1u;
+    uint32_t cnt = 0;  // the same in every lane of the group
+    for_each_row<T>(D, S, K, pi.x, pi.y, pi.z, [&](uint32_t j0, uint32_t j1) {
+        // the group runs the same iterations; a lane whose position is past
+        // the row takes no candidate
+        for (uint32_t base = j0; base < j1; base += 2 * G) {
+            const uint32_t ja = base + (uint32_t)l, jb = ja + G;
+            bool la = false, lb = false;
+            if (ja < j1) {
+                const V4<T> p = at(pos, ja);
+                const T dx = pi.x - p.x, dy = pi.y - p.y, dz = pi.z - p.z;
+                const T d2 = (dx * dx + dy * dy) + dz * dz;
+                if (d2 <= thr) {
+                    T q = h2 - d2;
+                    q = q > T(0) ? q : T(0);
+                    sum.add((q * q) * q);
+                    la = ja != self;
+                }
+            }
+            if (jb < j1) {
+                const V4<T> p = at(pos, jb);
+                const T dx = pi.x - p.x, dy = pi.y - p.y, dz = pi.z - p.z;
+                const T d2 = (dx * dx + dy * dy) + dz * dz;
+                if (d2 <= thr) {
+                    T q = h2 - d2;
+                    q = q > T(0) ? q : T(0);
+                    sum.add((q * q) * q);
+                    lb = jb != self;
+                }
+            }
+            // the force pass adds the particle's own term first: self is not listed
+            const uint32_t ga = (uint32_t)(__ballot(la) >> gfirst) & gmask;
+            const uint32_t gb = (uint32_t)(__ballot(lb) >> gfirst) & gmask;
+            const uint32_t na = __popc(ga);
+            if (la) {
+                const uint32_t m = cnt + __popc(ga & below);
+                if (m < cap) nbl[m] = ja;
+            }
+            if (lb) {
+                const uint32_t m = cnt + na + __popc(gb & below);
+                if (m < cap) nbl[m] = jb;
+            }
+            cnt += na + __popc(gb);
+        }
     });
-    D.nbc[slot * G + l] = cnt;
-    if (cnt > cap) atomicMax(D.ovf, cnt * (uint32_t)G);
     group_merge<G>(sum);
     if (l != 0) return;
+    D.nbc[slot] = cnt;
+    if (cnt > cap) atomicMax(D.ovf, cnt);
     const T rho = sum.value() * K.mc6;
     const T p = K.k * (rho - K.rho0);
     ((T*)D.rho)[slot] = rho;
@@ -650,9 +667,11 @@ struct ForceAcc {
     }
 };
 
-// Lane l's share of the force sums of one particle by walking its stencil
-// positions (inspection, and particles whose sub-list overflowed), with the
-// largest neighbour id and the neighbour count of that share.
+// Lane l's share of the force sums of one particle by walking its clipped
+// stencil rows (inspection, and particles whose list overflowed): every lane
+// visits all candidates in canonical order and takes the accepted
+// neighbours m = l, l+G, l+2G, ... (self excluded), as it would from the
+// list.  count and nbmax cover the lane's share (self counted by lane 0).
 template <typename T, bool COMP, int G>
 __device__ __forceinline__ void force_share_walk(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
                                                  int64_t slot, int l, const V4<T>& pi, const V4<T>& vi, T a_i,
@@ -663,18 +682,22 @@ __device__ __forceinline__ void force_share_walk(const DevState& D, const SceneD
     const V4<T>* __restrict__ vel = (const V4<T>*)D.s_vel;
     Terms<T> pend;
     bool has = false;  // pend holds the first of a pair
-    visit_strided<T, G>(D, S, K, pos, pi.x, pi.y, pi.z, l,
-                        [&](uint32_t j, T dx, T dy, T dz, T d2) {
-                            ++count;
-                            if ((int64_t)j == slot) return;  // own term: ForceAcc::self_term
-                            const int32_t idj = D.s_id[j];
-                            nbmax = idj > nbmax ? idj : nbmax;
-                            const V4<T> vj = vel[j];
-                            const Terms<T> t = pair_terms<T>(K, dx, dy, dz, d2, a_i, posa[j].w, vj.w, vi, vj);
-                            if (has) acc.add2(pend, t);
-                            else pend = t;
-                            has = !has;
-                        });
+    uint32_t m = 0;    // canonical index of the next accepted neighbour
+    if (l == 0) ++count;  // the particle itself
+    visit_neighbours<T>(D, S, K, pos, pi.x, pi.y, pi.z, [&](uint32_t j, T dx, T dy, T dz, T d2) {
+        if ((int64_t)j == slot) return;  // own term: ForceAcc::self_term
+        const bool mine = (m & (G - 1)) == (uint32_t)l;
+        ++m;
+        if (!mine) return;
+        ++count;
+        const int32_t idj = D.s_id[j];
+        nbmax = idj > nbmax ? idj : nbmax;
+        const V4<T> vj = at(vel, j);
+        const Terms<T> t = pair_terms<T>(K, dx, dy, dz, d2, a_i, at(posa, j).w, vj.w, vi, vj);
+        if (has) acc.add2(pend, t);
+        else pend = t;
+        has = !has;
+    });
     if (has) acc.add(pend);
 }
 
@@ -799,24 +822,25 @@ __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S
                                            int64_t slot, int l, const V4<T>& pi, const V4<T>& vi, T a_i,
                                            ForceAcc<T, COMP>& acc)
 {
-    const uint32_t cnt = D.nbc[slot * G + l];
-    if (cnt > (uint32_t)D.maxn / G) {
+    const uint32_t cnt = D.nbc[slot];
+    if (cnt > (uint32_t)D.maxn) {
         int32_t nbmax = -1, count = 0;
         force_share_walk<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc, nbmax, count);
         return;
     }
     const V4<T>* __restrict__ posa = (const V4<T>*)D.posa;
     const V4<T>* __restrict__ vel = (const V4<T>*)D.s_vel;
-    const uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn + l;
-    uint32_t k = 0;
-    // two neighbours per iteration: four independent gathers in flight
-    for (; k + 1 < cnt; k += 2) {
-        const uint32_t ja = nbl[k * G];
-        const uint32_t jb = nbl[(k + 1) * G];
-        const V4<T> pa = posa[ja];
-        const V4<T> va = vel[ja];
-        const V4<T> pb = posa[jb];
-        const V4<T> vb = vel[jb];
+    const uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn;
+    // lane l takes list entries l, l+G, l+2G, ...: the lanes of a particle
+    // differ by one neighbour at most; two per iteration, four gathers in flight
+    uint32_t k = (uint32_t)l;
+    for (; k + G < cnt; k += 2 * G) {
+        const uint32_t ja = nbl[k];
+        const uint32_t jb = nbl[k + G];
+        const V4<T> pa = at(posa, ja);
+        const V4<T> va = at(vel, ja);
+        const V4<T> pb = at(posa, jb);
+        const V4<T> vb = at(vel, jb);
         const T ax = pi.x - pa.x, ay = pi.y - pa.y, az = pi.z - pa.z;
         const T a2 = (ax * ax + ay * ay) + az * az;
         const T bx = pi.x - pb.x, by = pi.y - pb.y, bz = pi.z - pb.z;
@@ -825,9 +849,9 @@ __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S
                  pair_terms<T>(K, bx, by, bz, b2, a_i, pb.w, vb.w, vi, vb));
     }
     if (k < cnt) {
-        const uint32_t j = nbl[k * G];
-        const V4<T> pj = posa[j];
-        const V4<T> vj = vel[j];
+        const uint32_t j = nbl[k];
+        const V4<T> pj = at(posa, j);
+        const V4<T> vj = at(vel, j);
         const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const T d2 = (dx * dx + dy * dy) + dz * dz;
         acc.add(pair_terms<T>(K, dx, dy, dz, d2, a_i, pj.w, vj.w, vi, vj));

@@ -88,7 +88,7 @@ struct sphe_ctx {
 
     int block = 256;
     int remap = 1;
-    int maxn = 128;  // neighbour-list capacity per particle (grown when a step overflows it)
+    int maxn = 256;  // neighbour-list capacity per particle (grown when a step overflows it)
     int lanes = 4;   // lanes per particle in the density and force passes
     int sub_log2 = 2;  // x slabs per cell: 2^sub_log2
 
@@ -357,7 +357,9 @@ int build_layout(sphe_ctx* c)
         s.tcap = round_up((int64_t)s.tnx * s.tnz, SPHE_WAVE);
         TC += s.tcap;
     }
-    if (P > INT32_MAX / 2 || C > INT32_MAX / 2 || TC > INT32_MAX / 2)
+    // per-slot vectors are addressed with 32-bit byte offsets: each stays under 4 GiB
+    const int64_t vec_bytes = 4 * (int64_t)real_size(c->precision);
+    if (P > INT32_MAX / 2 || C > INT32_MAX / 2 || TC > INT32_MAX / 2 || P * vec_bytes > (int64_t)UINT32_MAX)
         return fail(c, SPHE_ERANGE, "context too large (slots %lld, cells %lld)", (long long)P, (long long)C);
     DevState& D = c->D;
     D.P = P;
@@ -400,7 +402,7 @@ int build_layout(sphe_ctx* c)
     DALLOC(D.tdq, Pn * 8);
     D.maxn = c->maxn;
     DALLOC(D.nbl, Pn * (size_t)D.maxn * 4);
-    DALLOC(D.nbc, Pn * 8 * 4);  // up to 8 lanes per particle
+    DALLOC(D.nbc, Pn * 4);
     DALLOC(D.ovf, 64);
     DALLOC(D.status, 64);
     DALLOC(c->d_mass_part, (size_t)sphe_mass_blocks() * 3 * 8);
@@ -738,16 +740,16 @@ int recount(sphe_ctx* c)
 }
 
 // The neighbour lists hold maxn entries per particle.  When a step found a
-// larger count (those lanes walked their stencil again, with the same
-// result), the lists grow for the following steps.
+// larger count (those particles walked their stencil again in the force
+// pass, with the same result), the lists grow for the following steps.
 int grow_lists(sphe_ctx* c)
 {
     uint32_t need = 0;
     HIPCHK(c, hipMemcpyAsync(&need, c->D.ovf, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (need <= (uint32_t)c->D.maxn) return SPHE_OK;
-    const int step = 16 * 8;
-    const int maxn = (int)(((int64_t)need * 5 / 4 + step - 1) / step * step);
+    const int step = 128;
+    const int maxn = (int)(((int64_t)need * 3 / 2 + step - 1) / step * step);
     void* p = nullptr;
     const size_t bytes = (size_t)c->D.P * (size_t)maxn * 4;
     hipError_t e = hipMalloc(&p, bytes);
@@ -780,10 +782,10 @@ int do_steps(sphe_ctx* c, int nsteps)
     }
     HIPCHK(c, hipGetLastError());
     const bool graph = c->use_graph && !c->profiling;
-    // A long call checks the list overflow flag every kList steps, so a
-    // state that compresses during the call does not keep walking its
-    // stencil to the end of the call.
-    const int kList = 64;
+    // A long call checks the list overflow flag every kList steps (one
+    // stream synchronisation each), so a state that compresses during the
+    // call does not keep walking its stencil to the end of the call.
+    const int kList = 16;
     for (int k0 = 0; k0 < nsteps; k0 += kList) {
         if (k0 > 0) {
             r = grow_lists(c);
