@@ -476,7 +476,7 @@ __global__ __launch_bounds__(256) void k_reorder(DevState D)
 
 // One G-lane group per particle (lane l = thread index mod G).  In every
 // clipped stencil row lane l takes the candidates at positions l, l+G, ...
-// (two per iteration) for its share of the density sum.  The group writes
+// (four per iteration) for its share of the density sum.  The group writes
 // one neighbour list per particle (self left out) in canonical order,
 // nbl[slot * maxn + m], m < maxn, count in nbc[slot]: a ballot over the
 // group orders each iteration's accepted candidates (the first of the pair
@@ -504,46 +504,40 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     const uint32_t gmask = (1u << G) - 1u, below = (1u << l) - 1u;
     uint32_t cnt = 0;  // the same in every lane of the group
     for_each_row<T>(D, S, K, pi.x, pi.y, pi.z, [&](uint32_t j0, uint32_t j1) {
-        // the group runs the same iterations; a lane whose position is past
-        // the row takes no candidate
-        for (uint32_t base = j0; base < j1; base += 2 * G) {
-            const uint32_t ja = base + (uint32_t)l, jb = ja + G;
-            bool la = false, lb = false;
-            if (ja < j1) {
-                const V4<T> p = at(pos, ja);
+        // the group runs the same iterations, U candidates per lane each;
+        // positions past the row load the row's last slot (all loads are
+        // issued before any use) and take nothing
+        constexpr int U = 4;
+        for (uint32_t base = j0; base < j1; base += U * G) {
+            uint32_t jj[U];
+            T d2[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                jj[u] = base + (uint32_t)(u * G + l);
+                const V4<T> p = at(pos, jj[u] < j1 ? jj[u] : j1 - 1);
                 const T dx = pi.x - p.x, dy = pi.y - p.y, dz = pi.z - p.z;
-                const T d2 = (dx * dx + dy * dy) + dz * dz;
-                if (d2 <= thr) {
-                    T q = h2 - d2;
+                d2[u] = (dx * dx + dy * dy) + dz * dz;
+            }
+            uint32_t g[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool in = (jj[u] < j1) & (d2[u] <= thr);
+                if (in) {
+                    T q = h2 - d2[u];
                     q = q > T(0) ? q : T(0);
                     sum.add((q * q) * q);
-                    la = ja != self;
                 }
+                // the force pass adds the particle's own term first: self is not listed
+                g[u] = (uint32_t)(__ballot(in & (jj[u] != self)) >> gfirst) & gmask;
             }
-            if (jb < j1) {
-                const V4<T> p = at(pos, jb);
-                const T dx = pi.x - p.x, dy = pi.y - p.y, dz = pi.z - p.z;
-                const T d2 = (dx * dx + dy * dy) + dz * dz;
-                if (d2 <= thr) {
-                    T q = h2 - d2;
-                    q = q > T(0) ? q : T(0);
-                    sum.add((q * q) * q);
-                    lb = jb != self;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if ((g[u] >> l) & 1u) {
+                    const uint32_t m = cnt + __popc(g[u] & below);
+                    if (m < cap) nbl[m] = jj[u];
                 }
+                cnt += __popc(g[u]);
             }
-            // the force pass adds the particle's own term first: self is not listed
-            const uint32_t ga = (uint32_t)(__ballot(la) >> gfirst) & gmask;
-            const uint32_t gb = (uint32_t)(__ballot(lb) >> gfirst) & gmask;
-            const uint32_t na = __popc(ga);
-            if (la) {
-                const uint32_t m = cnt + __popc(ga & below);
-                if (m < cap) nbl[m] = ja;
-            }
-            if (lb) {
-                const uint32_t m = cnt + na + __popc(gb & below);
-                if (m < cap) nbl[m] = jb;
-            }
-            cnt += na + __popc(gb);
         }
     });
     group_merge<G>(sum);
@@ -832,8 +826,30 @@ __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S
     const V4<T>* __restrict__ vel = (const V4<T>*)D.s_vel;
     const uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn;
     // lane l takes list entries l, l+G, l+2G, ...: the lanes of a particle
-    // differ by one neighbour at most; two per iteration, four gathers in flight
+    // differ by one neighbour at most; two pairs per iteration (eight
+    // gathers in flight), then one pair, then a single
     uint32_t k = (uint32_t)l;
+    for (; k + 3 * G < cnt; k += 4 * G) {
+        const uint32_t ja = nbl[k], jb = nbl[k + G], jc = nbl[k + 2 * G], jd = nbl[k + 3 * G];
+        const V4<T> pa = at(posa, ja), va = at(vel, ja), pb = at(posa, jb), vb = at(vel, jb);
+        const V4<T> pc = at(posa, jc), vc = at(vel, jc), pd = at(posa, jd), vd = at(vel, jd);
+        {
+            const T ax = pi.x - pa.x, ay = pi.y - pa.y, az = pi.z - pa.z;
+            const T a2 = (ax * ax + ay * ay) + az * az;
+            const T bx = pi.x - pb.x, by = pi.y - pb.y, bz = pi.z - pb.z;
+            const T b2 = (bx * bx + by * by) + bz * bz;
+            acc.add2(pair_terms<T>(K, ax, ay, az, a2, a_i, pa.w, va.w, vi, va),
+                     pair_terms<T>(K, bx, by, bz, b2, a_i, pb.w, vb.w, vi, vb));
+        }
+        {
+            const T ax = pi.x - pc.x, ay = pi.y - pc.y, az = pi.z - pc.z;
+            const T a2 = (ax * ax + ay * ay) + az * az;
+            const T bx = pi.x - pd.x, by = pi.y - pd.y, bz = pi.z - pd.z;
+            const T b2 = (bx * bx + by * by) + bz * bz;
+            acc.add2(pair_terms<T>(K, ax, ay, az, a2, a_i, pc.w, vc.w, vi, vc),
+                     pair_terms<T>(K, bx, by, bz, b2, a_i, pd.w, vd.w, vi, vd));
+        }
+    }
     for (; k + G < cnt; k += 2 * G) {
         const uint32_t ja = nbl[k];
         const uint32_t jb = nbl[k + G];
@@ -941,11 +957,12 @@ __device__ __forceinline__ void force_body(const DevState& D, int nblk, int rema
     count_position<T>(D, S, K, slot, p[0], p[1], p[2]);
 }
 
-// FP32: at most 80 registers (6 waves per SIMD): the list loop fits, and
-// only the rarely taken stencil walk of overflowing lanes spills.  Unbounded,
-// the walk raises the count to 101 and the occupancy of the whole kernel to 4.
+// FP32: at most 96 registers (5 waves per SIMD): the list loop fits, and
+// only the rarely taken stencil walk of overflowing particles spills.
+// Unbounded, the walk raises the count and the occupancy of the whole
+// kernel drops to 4.
 template <bool COMP, int G>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_force_f32(DevState D, int nblk,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_force_f32(DevState D, int nblk,
                                                                                               int remap)
 {
     force_body<float, COMP, G>(D, nblk, remap);
