@@ -223,9 +223,11 @@ int sphe_kernel_bytes(sphe_ctx* ctx, const char* kernel, double* bytes);
 
 /* ---- tuning ------------------------------------------------------------ */
 
-/* Launch parameters: block size of the neighbour kernels and the
- * XCD-contiguous block mapping.  sphe_autotune times the step for each
- * candidate and keeps the fastest. */
+/* Launch parameters: block size of the neighbour kernels (default 128) and
+ * the XCD-contiguous block mapping (default off: each XCD then takes every
+ * eighth block, which keeps the XCDs balanced when the neighbour counts vary
+ * across the domain).  sphe_autotune times the step for each block size
+ * with the mapping as set and keeps the fastest. */
 int sphe_set_tuning(sphe_ctx* ctx, int block, int xcd_remap);
 int sphe_get_tuning(const sphe_ctx* ctx, int* block, int* xcd_remap);
 /* Lanes per particle in the density and force passes (1, 2, 4 or 8;
@@ -240,6 +242,13 @@ int sphe_get_lanes(const sphe_ctx* ctx, int* lanes);
  * The slab count changes the canonical order, hence the summation order. */
 int sphe_set_subcells(sphe_ctx* ctx, int slabs);
 int sphe_get_subcells(const sphe_ctx* ctx, int* slabs);
+/* Neighbour-list entries per particle (default 256; a multiple of 16).  A
+ * particle with more neighbours continues its list in an overflow pool
+ * (up to twice the capacity) or, without room there, walks its stencil in
+ * the force pass; after any step that overflowed, the lists grow to 1.5x
+ * the largest count.  The capacity never changes results. */
+int sphe_set_list_capacity(sphe_ctx* ctx, int entries);
+int sphe_get_list_capacity(const sphe_ctx* ctx, int* entries);
 int sphe_autotune(sphe_ctx* ctx, int steps_per_trial);
 /* Saves the particle and terrain state of every scene (a host copy) and
  * restores it later; stepping a restored state repeats the same steps bit

@@ -86,5 +86,13 @@ struct DevState {
     uint32_t* nbl;
     uint32_t* nbc;
     int32_t maxn;
+    // overflow pool: a particle whose count passes maxn claims maxn more
+    // entries (pool[pbase[i] .. + maxn), neighbours maxn .. 2 maxn - 1) with
+    // an atomic on *ptop, which the scan of every step resets; pbase[i] is
+    // ~0u when it has none (the force pass then walks beyond maxn)
+    uint32_t* pool;
+    unsigned long long* ptop;
+    uint32_t* pbase;
+    int64_t pool_cap;
     uint32_t* ovf;  // largest neighbour count that overflowed a list
 };

@@ -380,11 +380,13 @@ __global__ __launch_bounds__(256) void k_scan_partials(uint32_t* __restrict__ pa
 }
 
 // start[i] = exclusive prefix of counts; counts are reset to zero for the
-// next step's arrivals, and so is the optional per-cell word array clear.
+// next step's arrivals, and so are the optional per-cell word array clear
+// and the optional single word zero.
 __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, uint32_t* __restrict__ start,
                                                    int64_t n, const uint32_t* __restrict__ partial,
-                                                   uint32_t* __restrict__ clear)
+                                                   uint32_t* __restrict__ clear, unsigned long long* __restrict__ zero)
 {
+    if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0ull;
     __shared__ uint32_t lds[4];
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
@@ -502,7 +504,9 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     const uint32_t self = (uint32_t)slot;
     const int gfirst = (int)(threadIdx.x & (SPHE_WAVE - 1)) & ~(G - 1);  // the group's first lane in the wave
     const uint32_t gmask = (1u << G) - 1u, below = (1u << l) - 1u;
-    uint32_t cnt = 0;  // the same in every lane of the group
+    uint32_t cnt = 0;      // the same in every lane of the group
+    uint32_t pb = ~0u;     // the group's overflow chunk in the pool (none yet)
+    bool claimed = false;  // the group asked for a chunk (at most once)
     for_each_row<T>(D, S, K, pi.x, pi.y, pi.z, [&](uint32_t j0, uint32_t j1) {
         // the group runs the same iterations, U candidates per lane each;
         // positions past the row load the row's last slot (all loads are
@@ -530,11 +534,25 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
                 // the force pass adds the particle's own term first: self is not listed
                 g[u] = (uint32_t)(__ballot(in & (jj[u] != self)) >> gfirst) & gmask;
             }
+            uint32_t total = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) total += __popc(g[u]);
+            if (cnt + total > cap && !claimed) {
+                // first pass over the list capacity: claim maxn pool entries
+                claimed = true;
+                uint32_t b0 = ~0u;
+                if (l == 0) {
+                    const unsigned long long t = atomicAdd(D.ptop, (unsigned long long)cap);
+                    b0 = t + cap <= (unsigned long long)D.pool_cap ? (uint32_t)t : ~0u;
+                }
+                pb = __shfl(b0, gfirst, SPHE_WAVE);
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if ((g[u] >> l) & 1u) {
                     const uint32_t m = cnt + __popc(g[u] & below);
                     if (m < cap) nbl[m] = jj[u];
+                    else if (pb != ~0u && m - cap < cap) D.pool[pb + (m - cap)] = jj[u];
                 }
                 cnt += __popc(g[u]);
             }
@@ -543,6 +561,7 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     group_merge<G>(sum);
     if (l != 0) return;
     D.nbc[slot] = cnt;
+    D.pbase[slot] = pb;
     if (cnt > cap) atomicMax(D.ovf, cnt);
     const T rho = sum.value() * K.mc6;
     const T p = K.k * (rho - K.rho0);
@@ -811,26 +830,20 @@ __device__ __forceinline__ int64_t erode_exchange(const SceneDev& S, const Scene
 // only.  Two neighbours per iteration keep four gathers in flight.  A lane
 // whose share overflowed the list walks its stencil positions instead (same
 // pairs, same order, same result).
-template <typename T, bool COMP, int G>
-__device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
-                                           int64_t slot, int l, const V4<T>& pi, const V4<T>& vi, T a_i,
-                                           ForceAcc<T, COMP>& acc)
+// Lane l's share of the force sums from list entries l, l+G, l+2G, ... of
+// a particle's neighbour sequence (fetch(k) = its k-th neighbour): the lanes
+// of a particle differ by one neighbour at most.  Two pairs per iteration
+// (eight gathers in flight), then one pair, then a single.
+template <typename T, bool COMP, int G, typename F>
+__device__ __forceinline__ void force_list(const DevState& D, const SceneConsts<T>& K, int l, uint32_t cnt,
+                                          const V4<T>& pi, const V4<T>& vi, T a_i, ForceAcc<T, COMP>& acc,
+                                          F&& fetch)
 {
-    const uint32_t cnt = D.nbc[slot];
-    if (cnt > (uint32_t)D.maxn) {
-        int32_t nbmax = -1, count = 0;
-        force_share_walk<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc, nbmax, count);
-        return;
-    }
     const V4<T>* __restrict__ posa = (const V4<T>*)D.posa;
     const V4<T>* __restrict__ vel = (const V4<T>*)D.s_vel;
-    const uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn;
-    // lane l takes list entries l, l+G, l+2G, ...: the lanes of a particle
-    // differ by one neighbour at most; two pairs per iteration (eight
-    // gathers in flight), then one pair, then a single
     uint32_t k = (uint32_t)l;
     for (; k + 3 * G < cnt; k += 4 * G) {
-        const uint32_t ja = nbl[k], jb = nbl[k + G], jc = nbl[k + 2 * G], jd = nbl[k + 3 * G];
+        const uint32_t ja = fetch(k), jb = fetch(k + G), jc = fetch(k + 2 * G), jd = fetch(k + 3 * G);
         const V4<T> pa = at(posa, ja), va = at(vel, ja), pb = at(posa, jb), vb = at(vel, jb);
         const V4<T> pc = at(posa, jc), vc = at(vel, jc), pd = at(posa, jd), vd = at(vel, jd);
         {
@@ -851,8 +864,8 @@ __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S
         }
     }
     for (; k + G < cnt; k += 2 * G) {
-        const uint32_t ja = nbl[k];
-        const uint32_t jb = nbl[k + G];
+        const uint32_t ja = fetch(k);
+        const uint32_t jb = fetch(k + G);
         const V4<T> pa = at(posa, ja);
         const V4<T> va = at(vel, ja);
         const V4<T> pb = at(posa, jb);
@@ -865,13 +878,39 @@ __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S
                  pair_terms<T>(K, bx, by, bz, b2, a_i, pb.w, vb.w, vi, vb));
     }
     if (k < cnt) {
-        const uint32_t j = nbl[k];
+        const uint32_t j = fetch(k);
         const V4<T> pj = at(posa, j);
         const V4<T> vj = at(vel, j);
         const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const T d2 = (dx * dx + dy * dy) + dz * dz;
         acc.add(pair_terms<T>(K, dx, dy, dz, d2, a_i, pj.w, vj.w, vi, vj));
     }
+}
+
+// Lane l's share of the force sums of one particle: from its list, the
+// list continued in its overflow chunk, or (no room for all neighbours)
+// by walking the stencil.
+template <typename T, bool COMP, int G>
+__device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
+                                           int64_t slot, int l, const V4<T>& pi, const V4<T>& vi, T a_i,
+                                           ForceAcc<T, COMP>& acc)
+{
+    const uint32_t cnt = D.nbc[slot];
+    const uint32_t cap = (uint32_t)D.maxn;
+    const uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn;
+    if (cnt <= cap) {
+        force_list<T, COMP, G>(D, K, l, cnt, pi, vi, a_i, acc, [&](uint32_t k) { return nbl[k]; });
+        return;
+    }
+    const uint32_t pb = D.pbase[slot];
+    if (pb != ~0u && cnt - cap <= cap) {
+        const uint32_t* __restrict__ pool = D.pool;
+        force_list<T, COMP, G>(D, K, l, cnt, pi, vi, a_i, acc,
+                               [&](uint32_t k) { return k < cap ? nbl[k] : pool[pb + (k - cap)]; });
+        return;
+    }
+    int32_t nbmax = -1, count = 0;
+    force_share_walk<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc, nbmax, count);
 }
 
 // One G-lane group per particle: the lanes sum their shares, merge them
@@ -1203,12 +1242,12 @@ void sphe_launch_hash(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent
 }
 
 void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, uint32_t* clear,
-                      hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+                      unsigned long long* zero, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     klaunch(k_scan_reduce, (unsigned)nb, 256, st, e0, nullptr, (const uint32_t*)cnt, n, partial);
     k_scan_partials<<<1, 256, 0, st>>>(partial, nb);
-    klaunch(k_scan_down, (unsigned)nb, 256, st, nullptr, e1, cnt, start, n, (const uint32_t*)partial, clear);
+    klaunch(k_scan_down, (unsigned)nb, 256, st, nullptr, e1, cnt, start, n, (const uint32_t*)partial, clear, zero);
 }
 
 int64_t sphe_scan_partials(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
@@ -1280,7 +1319,7 @@ void sphe_launch_two_phase(const DevState& D, hipStream_t st, hipEvent_t e0, hip
 {
     if (D.TC == 0 || D.P == 0) return;
     klaunch(k_tcount, grid_of(D.P, 256), 256, st, e0, nullptr, D);
-    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, nullptr, st);
+    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, nullptr, nullptr, st);
     k_tscatter<<<grid_of(D.P, 256), 256, 0, st>>>(D);
     klaunch(k_tapply, grid_of(D.TC, 256), 256, st, nullptr, e1, D);
 }

@@ -86,8 +86,8 @@ struct sphe_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<double, int>> prof;
 
-    int block = 256;
-    int remap = 1;
+    int block = 128;
+    int remap = 0;  // contiguous slot ranges per XCD: off (it unbalances the XCDs in non-uniform states)
     int maxn = 256;  // neighbour-list capacity per particle (grown when a step overflows it)
     int lanes = 4;   // lanes per particle in the density and force passes
     int sub_log2 = 2;  // x slabs per cell: 2^sub_log2
@@ -130,6 +130,14 @@ int fail(sphe_ctx* c, int code, const char* fmt, ...)
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 size_t real_size(int precision) { return precision == SPHE_FP64 ? sizeof(double) : sizeof(float); }
+
+// Overflow pool entries for list capacity maxn: room for an eighth of the
+// particles to hold twice the capacity (at least 64 of them).
+int64_t pool_entries(int64_t P, int maxn)
+{
+    const int64_t n = std::max<int64_t>(P / 8, 64) * maxn;
+    return std::min<int64_t>(n, (int64_t)INT32_MAX);
+}
 
 // ---------------------------------------------------------------- derive
 
@@ -403,6 +411,10 @@ int build_layout(sphe_ctx* c)
     D.maxn = c->maxn;
     DALLOC(D.nbl, Pn * (size_t)D.maxn * 4);
     DALLOC(D.nbc, Pn * 4);
+    D.pool_cap = pool_entries(P, D.maxn);
+    DALLOC(D.pool, (size_t)D.pool_cap * 4);
+    DALLOC(D.pbase, Pn * 4);
+    DALLOC(D.ptop, 64);
     DALLOC(D.ovf, 64);
     DALLOC(D.status, 64);
     DALLOC(c->d_mass_part, (size_t)sphe_mass_blocks() * 3 * 8);
@@ -431,6 +443,7 @@ int build_layout(sphe_ctx* c)
     HIPCHK(c, hipMemsetAsync(D.tcnt, 0, (size_t)(TC + 1) * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(D.status, 0, 64, c->stream));
     HIPCHK(c, hipMemsetAsync(D.ovf, 0, 64, c->stream));
+    HIPCHK(c, hipMemsetAsync(D.ptop, 0, 64, c->stream));
     if (TC > 0) HIPCHK(c, hipMemsetAsync(D.vol, 0, (size_t)TC * 8, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->layout_valid = true;
@@ -663,7 +676,7 @@ void launch_sort(sphe_ctx* c)
     const DevState& D = c->D;
     {
         Timed t(c, "scan");
-        sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, D.subw, c->stream, t.a, t.b);
+        sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, D.subw, D.ptop, c->stream, t.a, t.b);
     }
     {
         Timed t(c, "scatter");
@@ -700,10 +713,10 @@ void launch_step(sphe_ctx* c)
 std::string plan_signature(const sphe_ctx* c)
 {
     char b[256];
-    snprintf(b, sizeof(b), "%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%d/%d", c->precision, c->block, c->remap,
+    snprintf(b, sizeof(b), "%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%p/%d/%d", c->precision, c->block, c->remap,
              c->lanes, (long long)c->D.P, (long long)c->D.C, (long long)c->D.TC, (int)any_erosion(c, -1),
              (int)any_erosion(c, SPHE_EROSION_TWO_PHASE), (const void*)c->D.u_pos, (const void*)c->D.nbl,
-             c->D.maxn, c->D.sub_log2);
+             (const void*)c->D.pool, c->D.maxn, c->D.sub_log2);
     return b;
 }
 
@@ -742,6 +755,37 @@ int recount(sphe_ctx* c)
 // The neighbour lists hold maxn entries per particle.  When a step found a
 // larger count (those particles walked their stencil again in the force
 // pass, with the same result), the lists grow for the following steps.
+// New neighbour lists and overflow pool for capacity maxn (the old ones are
+// freed).  On an allocation failure the old ones stay: a particle without
+// room walks its stencil in the force pass, with the same result.
+int resize_lists(sphe_ctx* c, int maxn)
+{
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    void* p = nullptr;
+    void* q = nullptr;
+    const int64_t pool = pool_entries(c->D.P, maxn);
+    if (hipMalloc(&p, (size_t)c->D.P * (size_t)maxn * 4) != hipSuccess) return SPHE_OK;  // lists as they were
+    if (hipMalloc(&q, (size_t)pool * 4) != hipSuccess) {
+        (void)hipFree(p);
+        return SPHE_OK;
+    }
+    for (void*& a : c->allocs) {
+        if (a == (void*)c->D.nbl) {
+            (void)hipFree(a);
+            a = p;
+        } else if (a == (void*)c->D.pool) {
+            (void)hipFree(a);
+            a = q;
+        }
+    }
+    c->D.nbl = (uint32_t*)p;
+    c->D.pool = (uint32_t*)q;
+    c->D.pool_cap = pool;
+    c->D.maxn = maxn;
+    c->maxn = maxn;
+    return SPHE_OK;
+}
+
 int grow_lists(sphe_ctx* c)
 {
     uint32_t need = 0;
@@ -749,19 +793,8 @@ int grow_lists(sphe_ctx* c)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (need <= (uint32_t)c->D.maxn) return SPHE_OK;
     const int step = 128;
-    const int maxn = (int)(((int64_t)need * 3 / 2 + step - 1) / step * step);
-    void* p = nullptr;
-    const size_t bytes = (size_t)c->D.P * (size_t)maxn * 4;
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) return SPHE_OK;  // keep the walk for the overflowing lanes
-    for (void*& a : c->allocs)
-        if (a == (void*)c->D.nbl) {
-            (void)hipFree(a);
-            a = p;
-        }
-    c->D.nbl = (uint32_t*)p;
-    c->D.maxn = maxn;
-    c->maxn = maxn;
+    const int r = resize_lists(c, (int)(((int64_t)need * 3 / 2 + step - 1) / step * step));
+    if (r) return r;
     HIPCHK(c, hipMemsetAsync(c->D.ovf, 0, 4, c->stream));
     return SPHE_OK;
 }
@@ -1571,6 +1604,29 @@ int sphe_get_lanes(const sphe_ctx* c, int* lanes)
     return SPHE_OK;
 }
 
+int sphe_set_list_capacity(sphe_ctx* c, int entries)
+{
+    if (!c) return SPHE_EARG;
+    if (entries < 16 || entries > (1 << 20) || entries % 16)
+        return fail(c, SPHE_EARG, "list capacity must be a multiple of 16 in [16, 1048576]");
+    if (!c->layout_valid) {
+        c->maxn = entries;
+        return SPHE_OK;
+    }
+    if (entries == c->D.maxn) return SPHE_OK;
+    const int r = resize_lists(c, entries);
+    if (r) return r;
+    if (c->D.maxn != entries) return fail(c, SPHE_ENOMEM, "no device memory for lists of %d entries", entries);
+    return SPHE_OK;
+}
+
+int sphe_get_list_capacity(const sphe_ctx* c, int* entries)
+{
+    if (!c || !entries) return SPHE_EARG;
+    *entries = c->layout_valid ? c->D.maxn : c->maxn;
+    return SPHE_OK;
+}
+
 int sphe_set_subcells(sphe_ctx* c, int slabs)
 {
     if (!c) return SPHE_EARG;
@@ -1665,8 +1721,11 @@ int sphe_autotune(sphe_ctx* c, int steps_per_trial)
     int best_b = c->block, best_r = c->remap;
     double best = 1e30;
     hipEvent_t e0 = get_event(c), e1 = get_event(c);
+    // the block size only: the XCD-contiguous mapping (remap) wins on uniform
+    // states and loses by up to a third on non-uniform ones, which a trial on
+    // the current state cannot foresee, so it stays as set
     for (int b : {64, 128, 256, 512})
-        for (int rm : {0, 1}) {
+        for (int rm : {c->remap}) {
             c->block = b;
             c->remap = rm;
             c->device_valid = false;  // every trial starts from the same state

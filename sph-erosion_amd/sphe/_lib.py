@@ -97,6 +97,8 @@ SIGNATURES = {
     "sphe_get_lanes": (I, [P, C.POINTER(I)]),
     "sphe_save_state": (I, [P]),
     "sphe_restore_state": (I, [P]),
+    "sphe_set_list_capacity": (I, [P, I]),
+    "sphe_get_list_capacity": (I, [P, C.POINTER(I)]),
     "sphe_set_subcells": (I, [P, I]),
     "sphe_get_subcells": (I, [P, C.POINTER(I)]),
     "sphe_debug_cell_keys": (I, [P, I, P]),

@@ -264,6 +264,14 @@ class Context:
     def restore_state(self):
         self._chk(lib().sphe_restore_state(self._p), "restore_state")
 
+    def set_list_capacity(self, entries):
+        self._chk(lib().sphe_set_list_capacity(self._p, entries), "set_list_capacity")
+
+    def get_list_capacity(self):
+        v = C.c_int()
+        self._chk(lib().sphe_get_list_capacity(self._p, C.byref(v)), "get_list_capacity")
+        return v.value
+
     def set_subcells(self, slabs):
         self._chk(lib().sphe_set_subcells(self._p, slabs), "set_subcells")
 

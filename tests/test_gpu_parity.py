@@ -321,28 +321,51 @@ def test_crowded_cell_taken_whole():
     assert np.array_equal(gp, st["pos"]) and np.array_equal(gv, st["vel"])
 
 
-def test_list_overflow_walks_the_same_pairs():
-    """A pile-up beyond the neighbour-list capacity: the overflowing lanes walk
-    their stencil positions again, with the same result as the twin; the next
-    steps run with grown lists."""
-    rng = np.random.default_rng(4)
-    n = 3000
+def _blob(n=3000, seed=4):
+    rng = np.random.default_rng(seed)
     pos = (0.5 + rng.normal(0, 0.012, size=(n, 3))).astype(np.float32)  # dense blob
     p = sphe.default_params(h=0.03, mass=0.001, rest_density=1000.0, dt=1e-4, box_center=(0.5, 0.5, 0.5),
                             box_half=0.5, gravity=(0.0, -9.8, 0.0))
+    return pos, p
+
+
+def test_list_overflow_paths_match_the_twin():
+    """A blob with 32-entry lists: particles whose neighbours fit the list,
+    continue in an overflow chunk, find the pool exhausted, or pass twice the
+    capacity (stencil walk) all give the twin's bits."""
+    pos, p = _blob()
+    n = len(pos)
     ctx = sphe.Context()
     sid = ctx.add_scene(p)
     ctx.set_particles(sid, pos)
-    counts = ctx.debug_neighbours(sid, csr=False)[0]
-    assert counts.max() > 128  # beyond the initial list capacity
+    ctx.set_list_capacity(32)
+    assert ctx.get_list_capacity() == 32
+    counts = ctx.debug_neighbours(sid, csr=False)[0] - 1  # listed: self excluded
+    assert (counts <= 32).any() and ((counts > 32) & (counts <= 64)).any() and (counts > 64).any()
     osc = orc_scene_from(p)
-    ctx.step(3)
+    ctx.step(2)
     st = dict(pos=pos.astype(np.float64), vel=np.zeros((n, 3)), sed=np.zeros(n, np.int64))
-    for _ in range(3):
+    for _ in range(2):
         st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], np.arange(n, dtype=np.int32), real=4, comp=1)
     gp, gv = ctx.state_f64(sid)
     assert ctx.status() == 0
     assert np.array_equal(gp, st["pos"]) and np.array_equal(gv, st["vel"])
+    ctx.step(1)  # the overflow grew the lists for the following steps
+    assert ctx.get_list_capacity() >= 1.5 * counts.max()
+
+
+def test_list_capacity_never_changes_results():
+    pos, p = _blob(2000, 7)
+    out = []
+    for cap in (48, 96, 1024):
+        ctx = sphe.Context()
+        sid = ctx.add_scene(p)
+        ctx.set_particles(sid, pos)
+        ctx.set_list_capacity(cap)
+        ctx.step(3)
+        out.append(ctx.state_f64(sid))
+    for o in out[1:]:
+        assert np.array_equal(o[0], out[0][0]) and np.array_equal(o[1], out[0][1])
 
 
 def test_status_flags_nonfinite():
