@@ -133,7 +133,7 @@ def main():
     ctx = sphe.Context(sphe.FP32)
     sids = [ctx.add_synthetic(s) for s in scs]
     if not a.no_tune:
-        blk, rm = ctx.autotune(3)
+        blk, rm = ctx.autotune(5)
         log(f"bench: autotuned block={blk} xcd_remap={rm}")
     blk, rm = ctx.get_tuning()
     if a.trace:
@@ -198,7 +198,7 @@ def main():
                ms_per_step=1e3 * elapsed / a.steps, higher_is_better=True, scaling="weak",
                vs_baseline=None, dtype="fp32", data="synthetic",
                config=dict(workload=f"{a.workload}: {desc}", particles=N, scenes=len(scs),
-                           precision="fp32 compensated (Kahan)", block=blk, xcd_remap=rm),
+                           precision="fp32 compensated (Kahan)", block=blk, density_xcd_run=rm),
                roofline=roofline)
     out["value_kernel_by_kernel"] = N * a.steps / prof_elapsed
     out["replay_bit_identical"] = replay_identical

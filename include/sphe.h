@@ -224,10 +224,13 @@ int sphe_kernel_bytes(sphe_ctx* ctx, const char* kernel, double* bytes);
 /* ---- tuning ------------------------------------------------------------ */
 
 /* Launch parameters: block size of the neighbour kernels (default 128) and
- * the XCD-contiguous block mapping (default off: each XCD then takes every
- * eighth block, which keeps the XCDs balanced when the neighbour counts vary
- * across the domain).  sphe_autotune times the step for each block size
- * with the mapping as set and keeps the fastest. */
+ * the density pass's XCD mapping: blocks go to the 8 XCDs in runs of
+ * xcd_remap consecutive blocks (default 128: neighbouring cells share an
+ * XCD's L2; 0 or 1: plain round-robin order).  The force pass always uses
+ * the plain order, which keeps the XCDs balanced when neighbour counts vary
+ * across the domain.  sphe_autotune times the step for block sizes
+ * {64, 128, 256, 512} with the run as set and keeps the fastest.  Neither
+ * parameter changes results. */
 int sphe_set_tuning(sphe_ctx* ctx, int block, int xcd_remap);
 int sphe_get_tuning(const sphe_ctx* ctx, int* block, int* xcd_remap);
 /* Lanes per particle in the density and force passes (1, 2, 4 or 8;

@@ -46,14 +46,19 @@ __device__ __forceinline__ bool d_finite(float x) { return __builtin_isfinite(x)
 __device__ __forceinline__ bool d_finite(double x) { return __builtin_isfinite(x); }
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8).
-// Remap so that each XCD works on one contiguous range of particle slots:
-// neighbouring cells then share that XCD's L2.  Bijective for any count.
-__device__ __forceinline__ int xcd_block(int bid, int nblk, int remap)
+// Chunked remap: XCD x takes runs of `chunk` consecutive tiles (run c of
+// XCD x is tiles [(8c + x) chunk, (8c + x + 1) chunk)), so neighbouring
+// cells share that XCD's L2 while the runs of the 8 XCDs stay interleaved
+// over the domain (balanced when the work per particle varies).  chunk <= 1
+// is the plain order; blocks past the last complete set of 8 runs keep
+// their order.  Bijective for any count.
+__device__ __forceinline__ int xcd_block(int bid, int nblk, int chunk)
 {
-    if (!remap) return bid;
-    const int q = nblk >> 3, r = nblk & 7;
+    if (chunk <= 1) return bid;
+    const int span = 8 * chunk;
+    if (bid >= nblk / span * span) return bid;
     const int x = bid & 7, i = bid >> 3;
-    return x * q + (x < r ? x : r) + i;
+    return ((i / chunk) * 8 + x) * chunk + i % chunk;
 }
 
 __device__ __forceinline__ int wave_scene(const int32_t* chunk_scene, int64_t slot)

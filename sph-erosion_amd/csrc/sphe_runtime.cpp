@@ -87,7 +87,7 @@ struct sphe_ctx {
     std::map<std::string, std::pair<double, int>> prof;
 
     int block = 128;
-    int remap = 0;  // contiguous slot ranges per XCD: off (it unbalances the XCDs in non-uniform states)
+    int remap = 128;  // density: runs of 128 consecutive blocks per XCD (L2 locality); force: plain order
     int maxn = 256;  // neighbour-list capacity per particle (grown when a step overflows it)
     int lanes = 4;   // lanes per particle in the density and force passes
     int sub_log2 = 2;  // x slabs per cell: 2^sub_log2
@@ -698,7 +698,7 @@ void launch_step(sphe_ctx* c)
     }
     {
         Timed t(c, "force");
-        sphe_launch_force(D, c->block, c->remap, c->lanes, c->stream, t.a, t.b);
+        sphe_launch_force(D, c->block, 1, c->lanes, c->stream, t.a, t.b);  // plain XCD order: balance
     }
     if (any_erosion(c, SPHE_EROSION_TWO_PHASE)) {
         Timed t(c, "two_phase");
@@ -1584,8 +1584,9 @@ int sphe_set_tuning(sphe_ctx* c, int block, int remap)
     if (!c) return SPHE_EARG;
     if (block != 64 && block != 128 && block != 256 && block != 512)
         return fail(c, SPHE_EARG, "block must be 64, 128, 256 or 512");
+    if (remap < 0) return fail(c, SPHE_EARG, "xcd_remap (blocks per XCD run) must be >= 0");
     c->block = block;
-    c->remap = remap ? 1 : 0;
+    c->remap = remap;
     return SPHE_OK;
 }
 
@@ -1721,9 +1722,10 @@ int sphe_autotune(sphe_ctx* c, int steps_per_trial)
     int best_b = c->block, best_r = c->remap;
     double best = 1e30;
     hipEvent_t e0 = get_event(c), e1 = get_event(c);
-    // the block size only: the XCD-contiguous mapping (remap) wins on uniform
-    // states and loses by up to a third on non-uniform ones, which a trial on
-    // the current state cannot foresee, so it stays as set
+    // the block size only: the best XCD run length depends on how uneven the
+    // state is (long runs win on uniform states and lose by up to a fifth on
+    // pile-ups), which a trial on the current state cannot foresee; the run
+    // stays as set (default 128, near the best on both kinds of state)
     for (int b : {64, 128, 256, 512})
         for (int rm : {c->remap}) {
             c->block = b;

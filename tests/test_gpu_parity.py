@@ -237,7 +237,7 @@ def test_autotune_restores_state_and_counts():
     assert np.array_equal(p1, p2) and np.array_equal(v1, v2)
 
 
-@pytest.mark.parametrize("block,remap", [(64, 0), (128, 1), (512, 1)])
+@pytest.mark.parametrize("block,remap", [(64, 0), (64, 32), (128, 8), (256, 3), (512, 1)])
 def test_launch_shape_does_not_change_results(block, remap):
     sc = small_dam()
     c1, s1 = gpu_context(sc)
