@@ -481,14 +481,50 @@ __global__ __launch_bounds__(256) void k_reorder(DevState D)
 
 // --------------------------------------------------------------- density
 
+// Candidates per lane per iteration of the density pass (at most 4: one
+// byte each in the placement word).
+#define SPHE_DENSITY_UNROLL 4
+
+// Inclusive prefix sum of x over the G lanes of a group (lane l gets
+// x_0 + ... + x_l); *tot = the group's total.  The G lanes are consecutive
+// lanes of the wave, starting at a multiple of G.
+template <int G>
+__device__ __forceinline__ uint32_t group_prefix(uint32_t x, int l, uint32_t* tot)
+{
+    uint32_t y = x;
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+        const uint32_t o = __shfl_up(y, d, G);
+        if (l >= d) y += o;
+    }
+    *tot = __shfl(y, G - 1, G);
+    return y;
+}
+
+// G = 4: the groups are the wave's quads; three DPP quad permutes.
+template <>
+__device__ __forceinline__ uint32_t group_prefix<4>(uint32_t x, int l, uint32_t* tot)
+{
+    // quad_perm [0,0,1,2]: lane l reads lane l-1
+    const uint32_t s1 = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x90, 0xF, 0xF, false);
+    uint32_t y = x + (l >= 1 ? s1 : 0u);
+    // quad_perm [0,0,0,1]: lane l reads lane l-2
+    const uint32_t s2 = (uint32_t)__builtin_amdgcn_mov_dpp((int)y, 0x40, 0xF, 0xF, false);
+    y = y + (l >= 2 ? s2 : 0u);
+    // quad_perm [3,3,3,3]: the total, from lane 3
+    *tot = (uint32_t)__builtin_amdgcn_mov_dpp((int)y, 0xFF, 0xF, 0xF, false);
+    return y;
+}
+
+
 // One G-lane group per particle (lane l = thread index mod G).  In every
 // clipped stencil row lane l takes the candidates at positions l, l+G, ...
-// (four per iteration) for its share of the density sum.  The group writes
-// one neighbour list per particle (self left out) in canonical order,
-// nbl[slot * maxn + m], m < maxn, count in nbc[slot]: a ballot over the
-// group orders each iteration's accepted candidates (the first of the pair
-// in lane order, then the second).  The group then merges the partial sums
-// and lane 0 writes rho and (p / rho^2, m / rho).
+// (U = SPHE_DENSITY_UNROLL per iteration) for its share of the density sum.
+// The group writes one neighbour list per particle (self left out) in
+// canonical order, nbl[slot * maxn + m], m < maxn, count in nbc[slot]; a
+// byte-wise prefix sum over the group places each lane's entries.  The
+// group then merges the partial sums and lane 0 writes rho and
+// (p / rho^2, m / rho).
 template <typename T, bool COMP, int G>
 __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap)
 {
@@ -508,15 +544,14 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn;
     const uint32_t self = (uint32_t)slot;
     const int gfirst = (int)(threadIdx.x & (SPHE_WAVE - 1)) & ~(G - 1);  // the group's first lane in the wave
-    const uint32_t gmask = (1u << G) - 1u, below = (1u << l) - 1u;
     uint32_t cnt = 0;      // the same in every lane of the group
     uint32_t pb = ~0u;     // the group's overflow chunk in the pool (none yet)
     bool claimed = false;  // the group asked for a chunk (at most once)
     for_each_row<T>(D, S, K, pi.x, pi.y, pi.z, [&](uint32_t j0, uint32_t j1) {
-        // the group runs the same iterations, U candidates per lane each;
-        // positions past the row load the row's last slot (all loads are
-        // issued before any use) and take nothing
-        constexpr int U = 4;
+        // the group runs the same iterations; positions past the row load
+        // the row's last slot (all loads are issued before any use) and
+        // take nothing
+        constexpr int U = SPHE_DENSITY_UNROLL;
         for (uint32_t base = j0; base < j1; base += U * G) {
             uint32_t jj[U];
             T d2[U];
@@ -527,7 +562,8 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
                 const T dx = pi.x - p.x, dy = pi.y - p.y, dz = pi.z - p.z;
                 d2[u] = (dx * dx + dy * dy) + dz * dz;
             }
-            uint32_t g[U];
+            // byte u of x: 1 if this lane lists its u-th candidate
+            uint32_t x = 0;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const bool in = (jj[u] < j1) & (d2[u] <= thr);
@@ -537,30 +573,42 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
                     sum.add((q * q) * q);
                 }
                 // the force pass adds the particle's own term first: self is not listed
-                g[u] = (uint32_t)(__ballot(in & (jj[u] != self)) >> gfirst) & gmask;
+                x |= (uint32_t)(in & (jj[u] != self)) << (8 * u);
             }
-            uint32_t total = 0;
+            // canonical place of the lane's u-th candidate (position u*G + l
+            // of the block): the block's listed candidates at u' < u (all
+            // lanes), then those of lanes below l at u.  Byte-wise sums over
+            // the group (each byte stays <= 4 G, no carries): excl = lanes
+            // below l, tot = all lanes; tot * 0x01010100 puts the sum of
+            // bytes 0 .. u-1 into byte u.
+            uint32_t tot;
+            const uint32_t excl = group_prefix<G>(x, l, &tot) - x;
+            const uint32_t before = tot * 0x01010100u;
+            const uint32_t total = (tot * 0x01010101u) >> 24;
+            if (cnt + total <= cap) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) total += __popc(g[u]);
-            if (cnt + total > cap && !claimed) {
-                // first pass over the list capacity: claim maxn pool entries
-                claimed = true;
-                uint32_t b0 = ~0u;
-                if (l == 0) {
-                    const unsigned long long t = atomicAdd(D.ptop, (unsigned long long)cap);
-                    b0 = t + cap <= (unsigned long long)D.pool_cap ? (uint32_t)t : ~0u;
+                for (int u = 0; u < U; ++u)
+                    if ((x >> (8 * u)) & 1u) nbl[cnt + ((before >> (8 * u)) & 0xFFu) + ((excl >> (8 * u)) & 0xFFu)] = jj[u];
+            } else {
+                if (!claimed) {
+                    // first pass over the list capacity: claim maxn pool entries
+                    claimed = true;
+                    uint32_t b0 = ~0u;
+                    if (l == 0) {
+                        const unsigned long long t = atomicAdd(D.ptop, (unsigned long long)cap);
+                        b0 = t + cap <= (unsigned long long)D.pool_cap ? (uint32_t)t : ~0u;
+                    }
+                    pb = __shfl(b0, gfirst, SPHE_WAVE);
                 }
-                pb = __shfl(b0, gfirst, SPHE_WAVE);
-            }
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if ((g[u] >> l) & 1u) {
-                    const uint32_t m = cnt + __popc(g[u] & below);
-                    if (m < cap) nbl[m] = jj[u];
-                    else if (pb != ~0u && m - cap < cap) D.pool[pb + (m - cap)] = jj[u];
-                }
-                cnt += __popc(g[u]);
+                for (int u = 0; u < U; ++u)
+                    if ((x >> (8 * u)) & 1u) {
+                        const uint32_t m = cnt + ((before >> (8 * u)) & 0xFFu) + ((excl >> (8 * u)) & 0xFFu);
+                        if (m < cap) nbl[m] = jj[u];
+                        else if (pb != ~0u && m - cap < cap) D.pool[pb + (m - cap)] = jj[u];
+                    }
             }
+            cnt += total;
         }
     });
     group_merge<G>(sum);
