@@ -163,70 +163,100 @@ __device__ __forceinline__ T row_gap(T f)
 }
 
 // The stencil rows of a particle, clipped to what the smoothing sphere can
-// reach: calls row(j0, j1) with the slot range of each row (dz outer, dy
-// inner) that may hold a neighbour.  In cell units the radius is
-// h / cell < 1; a row whose y and z gaps g satisfy gy^2 + gz^2 > 1 is
-// skipped, and the others keep the x sub-cells that meet
-// [gx - w, gx + w], w = sqrt(1 - gy^2 - gz^2) + margin.  Every neighbour is
-// inside the range: the bounds are monotone in the position and include a
-// margin far above the rounding of the cell coordinates.  Cells with more
-// than 1023 particles carry no slab offsets and are taken whole.
+// reach.  Rows are numbered r = 3 (dz + 1) + (dy + 1) (dz outer, dy inner).
+// In cell units the radius is h / cell < 1; a row whose y and z gaps g
+// satisfy gy^2 + gz^2 > 1 is skipped, and the others keep the x sub-cells
+// that meet [gx - w, gx + w], w = sqrt(1 - gy^2 - gz^2) + margin.  Every
+// neighbour is inside the range: the bounds are monotone in the position
+// and include a margin far above the rounding of the cell coordinates.
+// Cells with more than 1023 particles carry no slab offsets and are taken
+// whole.
+template <typename T>
+struct RowProbe {
+    T gx, ylo, yhi, zlo, zhi, scale;
+    int kx, ky, kz, sl, smax, fmax, flo_min, fhi_max;
+};
+
+template <typename T>
+__device__ __forceinline__ RowProbe<T> row_probe(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
+                                                 T xi, T yi, T zi)
+{
+    RowProbe<T> P;
+    const int nx = S.dims[0], ny = S.dims[1], nz = S.dims[2];
+    P.gx = (xi - K.org[0]) * K.inv_cell;
+    const T gy = (yi - K.org[1]) * K.inv_cell;
+    const T gz = (zi - K.org[2]) * K.inv_cell;
+    P.kx = cidx<T>(d_floor(P.gx), nx - 1);
+    P.ky = cidx<T>(d_floor(gy), ny - 1);
+    P.kz = cidx<T>(d_floor(gz), nz - 1);
+    const T fy = gy - T(P.ky), fz = gz - T(P.kz);
+    P.ylo = row_gap<T>(fy);
+    P.yhi = row_gap<T>(T(1) - fy);
+    P.zlo = row_gap<T>(fz);
+    P.zhi = row_gap<T>(T(1) - fz);
+    P.sl = D.sub_log2;
+    P.smax = (1 << P.sl) - 1;
+    P.scale = T(1 << P.sl);
+    P.fmax = (nx << P.sl) - 1;
+    P.flo_min = (P.kx > 0 ? P.kx - 1 : 0) << P.sl;
+    P.fhi_max = ((P.kx < nx - 1 ? P.kx + 1 : nx - 1) << P.sl) + P.smax;
+    return P;
+}
+
+// Slot range [j0, j1) of row r; false when the row is outside the grid or
+// out of the sphere's reach.
+template <typename T>
+__device__ __forceinline__ bool row_range(const DevState& D, const SceneDev& S, const RowProbe<T>& P, int r,
+                                          uint32_t& j0, uint32_t& j1)
+{
+    const int nx = S.dims[0], ny = S.dims[1], nz = S.dims[2];
+    const int dzc = r / 3 - 1, dyc = r % 3 - 1;
+    const int z = P.kz + dzc, y = P.ky + dyc;
+    if (z < 0 || z >= nz || y < 0 || y >= ny) return false;
+    const T mz = dzc < 0 ? P.zlo : (dzc > 0 ? P.zhi : T(0));
+    const T my = dyc < 0 ? P.ylo : (dyc > 0 ? P.yhi : T(0));
+    const T r2 = my * my + mz * mz;
+    if (r2 > T(1)) return false;
+    const T w = d_sqrt(T(1) - r2) + T(SPHE_ROW_MARGIN);
+    int flo = cidx<T>(d_floor((P.gx - w) * P.scale), P.fmax);
+    int fhi = cidx<T>(d_floor((P.gx + w) * P.scale), P.fmax);
+    flo = flo > P.flo_min ? flo : P.flo_min;
+    fhi = fhi < P.fhi_max ? fhi : P.fhi_max;
+    const int64_t rk = (int64_t)S.cell_off + ((int64_t)z * ny + y) * nx;
+    const int sl = P.sl, smax = P.smax;
+    const int clo = flo >> sl, chi = fhi >> sl;
+    const uint32_t* __restrict__ start = D.start;
+    const uint32_t* __restrict__ subw = D.subw;
+    uint32_t a = start[rk + clo];
+    uint32_t b = start[rk + chi + 1];
+    if (sl) {
+        const int slo = flo & smax, shi = fhi & smax;
+        if (slo) {
+            const uint32_t w0 = subw[rk + clo];
+            if (!(w0 >> 31)) a += (w0 >> (10 * slo - 10)) & 1023u;
+        }
+        if (shi != smax) {
+            const uint32_t w1 = subw[rk + chi];
+            if (!(w1 >> 31)) b = start[rk + chi] + ((w1 >> (10 * shi)) & 1023u);
+        }
+    }
+    // rows of real cells lie inside [p_begin, p_begin + n); the clamp only
+    // matters if the counts were inconsistent
+    const uint32_t lo = (uint32_t)S.p_begin, hi = (uint32_t)(S.p_begin + S.n);
+    j0 = max(a, lo);
+    j1 = min(b, hi);
+    return true;
+}
+
+// Calls row(j0, j1) for each row that may hold a neighbour, in row order.
 template <typename T, typename F>
 __device__ __forceinline__ void for_each_row(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
                                              T xi, T yi, T zi, F&& row)
 {
-    const int nx = S.dims[0], ny = S.dims[1], nz = S.dims[2];
-    const T gx = (xi - K.org[0]) * K.inv_cell;
-    const T gy = (yi - K.org[1]) * K.inv_cell;
-    const T gz = (zi - K.org[2]) * K.inv_cell;
-    const int kx = cidx<T>(d_floor(gx), nx - 1);
-    const int ky = cidx<T>(d_floor(gy), ny - 1);
-    const int kz = cidx<T>(d_floor(gz), nz - 1);
-    const T fy = gy - T(ky), fz = gz - T(kz);
-    const T ylo = row_gap<T>(fy), yhi = row_gap<T>(T(1) - fy);
-    const T zlo = row_gap<T>(fz), zhi = row_gap<T>(T(1) - fz);
-    const int sl = D.sub_log2, smax = (1 << sl) - 1;
-    const T scale = T(1 << sl);
-    const int fmax = (nx << sl) - 1;
-    const int flo_min = (kx > 0 ? kx - 1 : 0) << sl;
-    const int fhi_max = ((kx < nx - 1 ? kx + 1 : nx - 1) << sl) + smax;
-    const uint32_t lo = (uint32_t)S.p_begin, hi = (uint32_t)(S.p_begin + S.n);
-    const uint32_t* __restrict__ start = D.start;
-    const uint32_t* __restrict__ subw = D.subw;
-    for (int dzc = -1; dzc <= 1; ++dzc) {
-        const int z = kz + dzc;
-        if (z < 0 || z >= nz) continue;
-        const T mz = dzc < 0 ? zlo : (dzc > 0 ? zhi : T(0));
-        for (int dyc = -1; dyc <= 1; ++dyc) {
-            const int y = ky + dyc;
-            if (y < 0 || y >= ny) continue;
-            const T my = dyc < 0 ? ylo : (dyc > 0 ? yhi : T(0));
-            const T r2 = my * my + mz * mz;
-            if (r2 > T(1)) continue;
-            const T w = d_sqrt(T(1) - r2) + T(SPHE_ROW_MARGIN);
-            int flo = cidx<T>(d_floor((gx - w) * scale), fmax);
-            int fhi = cidx<T>(d_floor((gx + w) * scale), fmax);
-            flo = flo > flo_min ? flo : flo_min;
-            fhi = fhi < fhi_max ? fhi : fhi_max;
-            const int64_t rk = (int64_t)S.cell_off + ((int64_t)z * ny + y) * nx;
-            const int clo = flo >> sl, chi = fhi >> sl;
-            uint32_t j0 = start[rk + clo];
-            uint32_t j1 = start[rk + chi + 1];
-            if (sl) {
-                const int slo = flo & smax, shi = fhi & smax;
-                if (slo) {
-                    const uint32_t w0 = subw[rk + clo];
-                    if (!(w0 >> 31)) j0 += (w0 >> (10 * slo - 10)) & 1023u;
-                }
-                if (shi != smax) {
-                    const uint32_t w1 = subw[rk + chi];
-                    if (!(w1 >> 31)) j1 = start[rk + chi] + ((w1 >> (10 * shi)) & 1023u);
-                }
-            }
-            // rows of real cells lie inside [p_begin, p_begin + n); the clamp
-            // only matters if the counts were inconsistent
-            row(max(j0, lo), min(j1, hi));
-        }
+    const RowProbe<T> P = row_probe<T>(D, S, K, xi, yi, zi);
+    for (int r = 0; r < 9; ++r) {
+        uint32_t j0, j1;
+        if (row_range<T>(D, S, P, r, j0, j1)) row(j0, j1);
     }
 }
 
@@ -517,9 +547,10 @@ __device__ __forceinline__ uint32_t group_prefix<4>(uint32_t x, int l, uint32_t*
 }
 
 
-// One G-lane group per particle (lane l = thread index mod G).  In every
-// clipped stencil row lane l takes the candidates at positions l, l+G, ...
-// (U = SPHE_DENSITY_UNROLL per iteration) for its share of the density sum.
+// One G-lane group per particle (lane l = thread index mod G).  The clipped
+// stencil rows form one stream of candidates; lane l takes stream positions
+// l, l+G, ... (U = SPHE_DENSITY_UNROLL per iteration) for its share of the
+// density sum.
 // The group writes one neighbour list per particle (self left out) in
 // canonical order, nbl[slot * maxn + m], m < maxn, count in nbc[slot]; a
 // byte-wise prefix sum over the group places each lane's entries.  The
@@ -547,18 +578,63 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     uint32_t cnt = 0;      // the same in every lane of the group
     uint32_t pb = ~0u;     // the group's overflow chunk in the pool (none yet)
     bool claimed = false;  // the group asked for a chunk (at most once)
-    for_each_row<T>(D, S, K, pi.x, pi.y, pi.z, [&](uint32_t j0, uint32_t j1) {
-        // the group runs the same iterations; positions past the row load
-        // the row's last slot (all loads are issued before any use) and
-        // take nothing
+    uint32_t stream_total = 0;
+    // The particle's clipped rows as one stream of candidates, t = 0 ..
+    // total-1 in canonical order.  Lane l computes rows l, l+G, ... and
+    // writes (first slot, length) to the group's row table in LDS.
+    __shared__ uint2 rowtab[512 / G * 9];
+    uint2* __restrict__ tab = rowtab + (threadIdx.x / G) * 9;
+    {
+        const RowProbe<T> P = row_probe<T>(D, S, K, pi.x, pi.y, pi.z);
+        uint32_t mine = 0;
+        for (int r = l; r < 9; r += G) {
+            uint32_t j0 = 0, j1 = 0;
+            const bool on = row_range<T>(D, S, P, r, j0, j1);
+            const uint32_t len = on && j1 > j0 ? j1 - j0 : 0u;
+            tab[r] = make_uint2(j0, len);
+            mine += len;
+        }
+        // the group's lanes are one wave's: order its LDS writes before the reads
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint32_t total;
+        (void)group_prefix<G>(mine, l, &total);
+        stream_total = total;
+    }
+    // Lane l takes stream positions l, l+G, ... (U = SPHE_DENSITY_UNROLL
+    // per iteration, blocks of U*G positions); its row pointer cur advances
+    // through the table as its position passes each row's end.  Positions
+    // past the stream load the particle itself and take nothing.
+    {
         constexpr int U = SPHE_DENSITY_UNROLL;
-        for (uint32_t base = j0; base < j1; base += U * G) {
+        const uint32_t total = stream_total;
+        int cur = -1;
+        uint32_t cur_end = 0, cur_off = 0;  // stream end of row cur; slot = t + cur_off inside it
+        for (uint32_t base = 0; base < total; base += U * G) {
             uint32_t jj[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t t = base + (uint32_t)(u * G + l);
+                ok[u] = t < total;
+                jj[u] = self;
+                if (ok[u]) {
+                    while (t >= cur_end && cur < 8) {
+                        const uint2 e = tab[++cur];
+                        cur_off = e.x - cur_end;
+                        cur_end += e.y;
+                    }
+                    // the table covers the stream; a position it does not
+                    // cover (never, with consistent counts) takes nothing
+                    ok[u] = t < cur_end;
+                    if (ok[u]) jj[u] = t + cur_off;
+                }
+            }
             T d2[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                jj[u] = base + (uint32_t)(u * G + l);
-                const V4<T> p = at(pos, jj[u] < j1 ? jj[u] : j1 - 1);
+                const V4<T> p = at(pos, jj[u]);
                 const T dx = pi.x - p.x, dy = pi.y - p.y, dz = pi.z - p.z;
                 d2[u] = (dx * dx + dy * dy) + dz * dz;
             }
@@ -566,7 +642,7 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
             uint32_t x = 0;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const bool in = (jj[u] < j1) & (d2[u] <= thr);
+                const bool in = ok[u] & (d2[u] <= thr);
                 if (in) {
                     T q = h2 - d2[u];
                     q = q > T(0) ? q : T(0);
@@ -584,8 +660,8 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
             uint32_t tot;
             const uint32_t excl = group_prefix<G>(x, l, &tot) - x;
             const uint32_t before = tot * 0x01010100u;
-            const uint32_t total = (tot * 0x01010101u) >> 24;
-            if (cnt + total <= cap) {
+            const uint32_t listed = (tot * 0x01010101u) >> 24;
+            if (cnt + listed <= cap) {
 #pragma unroll
                 for (int u = 0; u < U; ++u)
                     if ((x >> (8 * u)) & 1u) nbl[cnt + ((before >> (8 * u)) & 0xFFu) + ((excl >> (8 * u)) & 0xFFu)] = jj[u];
@@ -608,9 +684,9 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
                         else if (pb != ~0u && m - cap < cap) D.pool[pb + (m - cap)] = jj[u];
                     }
             }
-            cnt += total;
+            cnt += listed;
         }
-    });
+    }
     group_merge<G>(sum);
     if (l != 0) return;
     D.nbc[slot] = cnt;
