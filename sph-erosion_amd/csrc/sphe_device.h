@@ -66,7 +66,7 @@ struct DevState {
     void* s_pos; void* s_vel; int64_t* s_sed; int32_t* s_id;
     void* rho; void* posa;              // density; (x, y, z, p / rho^2) per sorted slot
     int32_t* key; uint32_t* arank;      // cell key << sub_log2 | x slab, arrival rank in the cell
-    int32_t* tmp_id; int32_t* tmp_key; int32_t* tmp_src;
+    uint64_t* tmp_kid; int32_t* tmp_src;  // scatter output: (key, id) packed, previous slot
     uint32_t* cnt; uint32_t* start; uint32_t* partial; int64_t npartial;
     // cells are split into 2^sub_log2 slabs along x (sub_log2 <= 2); subw
     // holds each cell's slab offsets (10 bits per inner boundary; bit 31:

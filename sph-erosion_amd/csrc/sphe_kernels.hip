@@ -2,12 +2,14 @@
 //
 // One step (all scenes of a context at once), reference Erosion/fluid_system.h
 // FluidSystemSPH::Run (:104-183) + advance (:306-353):
-//   scan      counts -> cell starts (3 kernels; also re-zeroes the counts)
+//   scan      counts -> cell starts (2 kernels; also re-zeroes the counts)
 //   scatter   each particle to its cell segment (arrival order)
-//   reorder   rank inside the cell by particle id -> canonical (cell, id)
-//             order; gathers the state into sorted SoA buffers
-//   density   poly6 density + pressure terms over the 27-cell stencil
-//             (fluid_system.h:109-124)
+//   reorder   rank inside the cell by (x slab, particle id) -> canonical
+//             (cell, slab, id) order; gathers the state into sorted SoA
+//             buffers and writes the cell's slab offsets
+//   density   poly6 density + pressure terms over the stencil rows clipped
+//             to the smoothing sphere (fluid_system.h:109-124), and the
+//             neighbour lists for the force pass
 //   force     pressure, viscosity, colour normal/Laplacian (:129-178),
 //             symplectic Euler + box response (:318-350, :355-407),
 //             heightfield collision and erosion exchange (build-defined),
@@ -400,20 +402,6 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(256) void k_scan_partials(uint32_t* __restrict__ partial, int64_t np)
-{
-    __shared__ uint32_t lds[4];
-    uint32_t carry = 0;
-    for (int64_t b = 0; b < np; b += 256) {
-        const int64_t i = b + threadIdx.x;
-        uint32_t v = i < np ? partial[i] : 0u;
-        uint32_t tot;
-        uint32_t ex = block_exclusive_scan(v, &tot, lds);
-        if (i < np) partial[i] = carry + ex;
-        carry += tot;
-    }
-}
-
 // start[i] = exclusive prefix of counts; counts are reset to zero for the
 // next step's arrivals, and so are the optional per-cell word array clear
 // and the optional single word zero.
@@ -422,7 +410,7 @@ __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, u
                                                    uint32_t* __restrict__ clear, unsigned long long* __restrict__ zero)
 {
     if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0ull;
-    __shared__ uint32_t lds[4];
+    __shared__ uint32_t lds[8];
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     load_run(cnt, base, n, v);
@@ -430,7 +418,13 @@ __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, u
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) s += v[i];
     uint32_t tot;
-    uint32_t run = block_exclusive_scan(s, &tot, lds) + partial[blockIdx.x];
+    uint32_t run = block_exclusive_scan(s, &tot, lds);
+    // the totals of the tiles before this one
+    uint32_t prior = 0;
+    for (uint32_t q = threadIdx.x; q < blockIdx.x; q += blockDim.x) prior += partial[q];
+    uint32_t ptot;
+    (void)block_exclusive_scan(prior, &ptot, lds + 4);
+    run += ptot;
     uint32_t z[SCAN_ITEMS];
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
@@ -459,8 +453,8 @@ __global__ __launch_bounds__(256) void k_scatter(DevState D)
         atomicOr(D.status, 2u);
         return;
     }
-    D.tmp_id[d] = D.u_id[p];
-    D.tmp_key[d] = key;
+    // (key, id) as one word that compares as the pair: id's sign bit flipped
+    D.tmp_kid[d] = ((uint64_t)(uint32_t)key << 32) | (uint32_t)(D.u_id[p] ^ (int32_t)0x80000000);
     D.tmp_src[d] = (int32_t)p;
 }
 
@@ -475,7 +469,8 @@ __global__ __launch_bounds__(256) void k_reorder(DevState D)
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= D.P) return;
     const int sl = D.sub_log2;
-    const int32_t fk = D.tmp_key[d];
+    const uint64_t kid = D.tmp_kid[d];
+    const int32_t fk = (int32_t)(kid >> 32);
     const int32_t c = fk >> sl;
     if (fk < 0 || (int64_t)c >= D.C) {
         atomicOr(D.status, 2u);
@@ -483,7 +478,7 @@ __global__ __launch_bounds__(256) void k_reorder(DevState D)
     }
     const uint32_t s0 = D.start[c];
     const uint32_t s1 = (int64_t)D.start[c + 1] <= D.P ? D.start[c + 1] : (uint32_t)D.P;
-    const int32_t my = D.tmp_id[d];
+    const int32_t my = (int32_t)((uint32_t)kid ^ 0x80000000u);
     const int32_t src = D.tmp_src[d];
     // one count field per slab boundary: a particle in slab k adds 1 to the
     // fields of the boundaries s > k
@@ -491,11 +486,9 @@ __global__ __launch_bounds__(256) void k_reorder(DevState D)
     const uint32_t smask = (1u << sl) - 1u;
     uint32_t rank = 0, below = 0;
     for (uint32_t j = s0; j < s1; ++j) {
-        const int32_t ok = D.tmp_key[j];
-        const int32_t o = D.tmp_id[j];
-        const bool before = ok < fk || (ok == fk && (o < my || (o == my && D.tmp_src[j] < src)));
-        rank += before ? 1u : 0u;
-        below += fields & (~0u << (10u * ((uint32_t)ok & smask)));
+        const uint64_t o = D.tmp_kid[j];
+        rank += (o < kid || (o == kid && D.tmp_src[j] < src)) ? 1u : 0u;
+        below += fields & (~0u << (10u * ((uint32_t)(o >> 32) & smask)));
     }
     const uint32_t f = s0 + rank;
     if ((int64_t)f >= D.P || src < 0 || (int64_t)src >= D.P) {
@@ -1375,7 +1368,6 @@ void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* parti
 {
     const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     klaunch(k_scan_reduce, (unsigned)nb, 256, st, e0, nullptr, (const uint32_t*)cnt, n, partial);
-    k_scan_partials<<<1, 256, 0, st>>>(partial, nb);
     klaunch(k_scan_down, (unsigned)nb, 256, st, nullptr, e1, cnt, start, n, (const uint32_t*)partial, clear, zero);
 }
 

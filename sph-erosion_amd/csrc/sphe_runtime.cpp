@@ -389,8 +389,7 @@ int build_layout(sphe_ctx* c)
     DALLOC(D.posa, Pn * 4 * E);
     DALLOC(D.key, Pn * 4);
     DALLOC(D.arank, Pn * 4);
-    DALLOC(D.tmp_id, Pn * 4);
-    DALLOC(D.tmp_key, Pn * 4);
+    DALLOC(D.tmp_kid, Pn * 8);
     DALLOC(D.tmp_src, Pn * 4);
     DALLOC(D.cnt, (size_t)(C + 1) * 4);
     DALLOC(D.start, (size_t)(C + 1) * 4);
