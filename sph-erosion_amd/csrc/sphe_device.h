@@ -70,7 +70,9 @@ struct DevState {
     uint32_t* cnt; uint32_t* start; uint32_t* partial; int64_t npartial;
     // cells are split into 2^sub_log2 slabs along x (sub_log2 <= 2); subw
     // holds each cell's slab offsets (10 bits per inner boundary; bit 31:
-    // more than 1023 particles, no offsets)
+    // more than 1023 particles, no offsets), rewritten every step for the
+    // occupied cells (an empty cell's word may be stale: readers clamp the
+    // offsets to the cell)
     uint32_t* subw; int32_t sub_log2;
     int64_t* vol; float* height;
     int64_t* dq; int32_t* tcell;        // two-phase erosion records

@@ -233,13 +233,15 @@ __device__ __forceinline__ bool row_range(const DevState& D, const SceneDev& S, 
     uint32_t b = start[rk + chi + 1];
     if (sl) {
         const int slo = flo & smax, shi = fhi & smax;
+        // the offsets are clamped to the cell: an empty cell's word is not
+        // rewritten each step, and any value then gives its empty range
         if (slo) {
             const uint32_t w0 = subw[rk + clo];
-            if (!(w0 >> 31)) a += (w0 >> (10 * slo - 10)) & 1023u;
+            if (!(w0 >> 31)) a = min(a + ((w0 >> (10 * slo - 10)) & 1023u), start[rk + clo + 1]);
         }
         if (shi != smax) {
             const uint32_t w1 = subw[rk + chi];
-            if (!(w1 >> 31)) b = start[rk + chi] + ((w1 >> (10 * shi)) & 1023u);
+            if (!(w1 >> 31)) b = min(start[rk + chi] + ((w1 >> (10 * shi)) & 1023u), start[rk + chi + 1]);
         }
     }
     // rows of real cells lie inside [p_begin, p_begin + n); the clamp only
@@ -402,12 +404,13 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// start[i] = exclusive prefix of counts; counts are reset to zero for the
-// next step's arrivals, and so are the optional per-cell word array clear
-// and the optional single word zero.
+// start[i] = exclusive prefix of counts.  With zero_counts the counts are
+// reset to zero for the next arrivals (the particle sort instead has each
+// particle's scatter reset its own cell); the optional single word zero is
+// reset too.
 __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, uint32_t* __restrict__ start,
                                                    int64_t n, const uint32_t* __restrict__ partial,
-                                                   uint32_t* __restrict__ clear, unsigned long long* __restrict__ zero)
+                                                   int zero_counts, unsigned long long* __restrict__ zero)
 {
     if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0ull;
     __shared__ uint32_t lds[8];
@@ -434,8 +437,7 @@ __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, u
         z[i] = 0u;
     }
     store_run(start, base, n, v);
-    store_run(cnt, base, n, z);
-    if (clear) store_run(clear, base, n, z);
+    if (zero_counts) store_run(cnt, base, n, z);
 }
 
 __global__ __launch_bounds__(256) void k_scatter(DevState D)
@@ -449,6 +451,7 @@ __global__ __launch_bounds__(256) void k_scatter(DevState D)
         return;
     }
     const uint32_t d = D.start[c] + D.arank[p];
+    D.cnt[c] = 0u;  // the cell's count is used up: reset for the next step's arrivals
     if ((int64_t)d >= D.P) {  // counts inconsistent with the slots: report, never write out of range
         atomicOr(D.status, 2u);
         return;
@@ -1363,12 +1366,13 @@ void sphe_launch_hash(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent
     else klaunch(k_hash_count<float>, grid_of(D.P, 256), 256, st, e0, e1, D);
 }
 
-void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, uint32_t* clear,
+void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, bool zero_counts,
                       unsigned long long* zero, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     klaunch(k_scan_reduce, (unsigned)nb, 256, st, e0, nullptr, (const uint32_t*)cnt, n, partial);
-    klaunch(k_scan_down, (unsigned)nb, 256, st, nullptr, e1, cnt, start, n, (const uint32_t*)partial, clear, zero);
+    klaunch(k_scan_down, (unsigned)nb, 256, st, nullptr, e1, cnt, start, n, (const uint32_t*)partial,
+            zero_counts ? 1 : 0, zero);
 }
 
 int64_t sphe_scan_partials(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
@@ -1440,7 +1444,7 @@ void sphe_launch_two_phase(const DevState& D, hipStream_t st, hipEvent_t e0, hip
 {
     if (D.TC == 0 || D.P == 0) return;
     klaunch(k_tcount, grid_of(D.P, 256), 256, st, e0, nullptr, D);
-    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, nullptr, nullptr, st);
+    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, true, nullptr, st);
     k_tscatter<<<grid_of(D.P, 256), 256, 0, st>>>(D);
     klaunch(k_tapply, grid_of(D.TC, 256), 256, st, nullptr, e1, D);
 }

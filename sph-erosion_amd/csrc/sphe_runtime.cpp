@@ -675,7 +675,7 @@ void launch_sort(sphe_ctx* c)
     const DevState& D = c->D;
     {
         Timed t(c, "scan");
-        sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, D.subw, D.ptop, c->stream, t.a, t.b);
+        sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, false, D.ptop, c->stream, t.a, t.b);
     }
     {
         Timed t(c, "scatter");
