@@ -189,6 +189,13 @@ int sphe_fixed_exponent(const sphe_ctx* ctx, int scene, int* e);
  * Grid::genIndices (grid.h:118-136).  Pass NULL to query sizes. */
 int sphe_terrain_mesh(sphe_ctx* ctx, int scene, float* verts, size_t* nverts_floats,
                       uint32_t* indices, size_t* nindices);
+/* The same vertex stream left in device memory (computed by a kernel from
+ * the current heights) for a renderer that copies it into its vertex buffer
+ * through graphics interop: *dverts stays valid until the next call that
+ * steps, edits or destroys the context, or asks for another mesh.  Replaces
+ * the host round trip of Grid::UpdateGrid + glBufferData
+ * (grid.h:138-176, main.cpp:228-240). */
+int sphe_terrain_mesh_device(sphe_ctx* ctx, int scene, const float** dverts, size_t* nfloats);
 
 /* ---- stepping ---------------------------------------------------------- */
 

@@ -104,6 +104,9 @@ def lib():
         L.orc_heights.argtypes = [C.POINTER(OrcConsts), C.c_int, P, P]
         L.orc_erode.argtypes = [C.POINTER(OrcScene), C.c_int, C.c_int, P, P, P, P, P]
         L.orc_erode.restype = C.c_int
+        L.orc_terrain_mesh.argtypes = [P, C.c_int, C.c_int, C.c_float, P, P]
+        L.orc_terrain_indices.argtypes = [C.c_int, C.c_int, P]
+        L.orc_terrain_indices.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -120,6 +123,9 @@ def reflib():
         R.refx_lattice.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
         R.refx_lattice.restype = C.c_int
         R.refx_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+        R.refx_grid_mesh.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                     C.POINTER(C.c_int)]
+        R.refx_grid_mesh.restype = C.c_int
         _ref = R
     return _ref
 
@@ -225,6 +231,37 @@ def neighbours_brute(pos, thr, csr=True):
     idx = np.zeros(max(total, 1), dtype=np.int32)
     lib().orc_neighbours_brute(_ptr(pos), n, float(thr), _ptr(counts), _ptr(off), _ptr(idx))
     return counts, off, idx[:total]
+
+
+def terrain_mesh(heights, cell, origin):
+    """Grid::UpdateGrid vertex stream (6 floats per cell) of a float heightfield [nz, nx]."""
+    H = np.ascontiguousarray(heights, dtype=np.float32)
+    nz, nx = H.shape
+    org = np.asarray(origin, dtype=np.float32)
+    out = np.zeros(nx * nz * 6, dtype=np.float32)
+    lib().orc_terrain_mesh(_ptr(H), nx, nz, float(np.float32(cell)), _ptr(org), _ptr(out))
+    return out
+
+
+def terrain_indices(nx, nz):
+    n = lib().orc_terrain_indices(nx, nz, None)
+    out = np.zeros(n, dtype=np.uint32)
+    lib().orc_terrain_indices(nx, nz, _ptr(out))
+    return out
+
+
+def refbin_grid_mesh(img, d):
+    """The reference's Grid(d, d, d) UpdateGrid on the 512 x 512 uint8 image img
+    (indexed img[x, z], as GetHeightfieldAt): (vertex floats, indices)."""
+    R = reflib()
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    assert img.shape == (512, 512)
+    nv = C.c_int()
+    ni = R.refx_grid_mesh(_ptr(img), d, None, 0, None, 0, C.byref(nv))
+    v = np.zeros(nv.value, dtype=np.float32)
+    i = np.zeros(ni, dtype=np.uint32)
+    R.refx_grid_mesh(_ptr(img), d, _ptr(v), nv.value, _ptr(i), ni, C.byref(nv))
+    return v, i
 
 
 def heights(consts, vol_fx):

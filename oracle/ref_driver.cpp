@@ -110,4 +110,24 @@ void refx_step(void* recs, int n, const float* prm, int steps)
     std::memcpy(recs, P.data(), sizeof(FluidParticle) * n);
 }
 
+// The reference's terrain mesh: a Grid(d, d, d) loads the 512 x 512 uint8
+// heightfield img (Grid::LoadHeightfield, grid.h:98-102) and runs
+// Grid::UpdateGrid(d, d, d) (grid.h:138-176, with genIndices :118-136).
+// Copies up to vcap floats of GetSurfaceParts and icap indices of
+// GetIndices (grid.h:807-830); returns the index count, *nv the float count.
+int refx_grid_mesh(const unsigned char* img, int d, float* verts, int vcap, unsigned int* idx, int icap, int* nv)
+{
+    Grid* g = new Grid(d, d, d);
+    g->LoadHeightfield(const_cast<unsigned char*>(img));
+    g->UpdateGrid(d, d, d);
+    const std::vector<float> v = g->GetSurfaceParts();
+    const std::vector<unsigned int> i = g->GetIndices();
+    *nv = (int)v.size();
+    if (verts && vcap >= (int)v.size()) std::memcpy(verts, v.data(), v.size() * sizeof(float));
+    if (idx && icap >= (int)i.size()) std::memcpy(idx, i.data(), i.size() * sizeof(unsigned int));
+    // Grid has no destructor (grid.h: raw arrays are never freed); the test
+    // process is short-lived
+    return (int)i.size();
+}
+
 }  // extern "C"

@@ -22,6 +22,10 @@
  *   orc_step terrain   build-defined heightfield collision and erosion (the
  *   and erosion        reference has none).  Pinned by closed-form single
  *                      particle cases and exact integer mass conservation.
+ *   orc_terrain_mesh / grid.h:118-176 (UpdateGrid, genIndices) on a float
+ *   orc_terrain_indices heightfield in world units.  Pinned bit-for-bit
+ *                      against oracle/_ref (the reference's Grid) and the
+ *                      fixture mesh_ref in voxel units (cell 1).
  */
 #include "sph_oracle.h"
 
@@ -379,6 +383,75 @@ void orc_cell_keys(const orc_consts* c, const float* pos3, int n, int32_t* keys)
         }
         keys[i] = (k[2] * c->dims[1] + k[1]) * c->dims[0] + k[0];
     }
+}
+
+/* Grid::UpdateGrid's vertex stream (grid.h:138-176) for a heightfield of
+ * nx x nz cells, row z: (x, y, z, normal) per cell, with x and z at the
+ * cell centres org + (i + 0.5) cell and y the height.  The normal is the
+ * reference's normalize(cross(u,l) + cross(u,r) + cross(d,l) + cross(d,r))
+ * over the vectors to the four edge neighbours present (normalize =
+ * v * (1 / sqrt(dot(v, v))), glm func_geometric.inl and
+ * func_exponential.inl:136-139); a zero sum gives a zero normal.  With
+ * cell 1 and org -0.5 it is the reference's stream on integer heights. */
+void orc_terrain_mesh(const float* H, int nx, int nz, float cell, const float* org, float* out)
+{
+    size_t w = 0;
+    for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nx; x++) {
+            const float y = H[(size_t)z * nx + x];
+            float u[3] = {0, 0, 0}, d[3] = {0, 0, 0}, r[3] = {0, 0, 0}, l[3] = {0, 0, 0};
+            if (x - 1 >= 0) { l[0] = cell; l[1] = y - H[(size_t)z * nx + x - 1]; }
+            if (x + 1 < nx) { r[0] = cell; r[1] = H[(size_t)z * nx + x + 1] - y; }
+            if (z - 1 >= 0) { u[1] = y - H[(size_t)(z - 1) * nx + x]; u[2] = cell; }
+            if (z + 1 < nz) { d[1] = H[(size_t)(z + 1) * nx + x] - y; d[2] = cell; }
+            const float* A[4] = {u, u, d, d};
+            const float* B[4] = {l, r, l, r};
+            float n[3] = {0, 0, 0};
+            for (int k = 0; k < 4; k++) {
+                const float* a = A[k];
+                const float* b = B[k];
+                const float c0 = a[1] * b[2] - a[2] * b[1];
+                const float c1 = a[2] * b[0] - a[0] * b[2];
+                const float c2 = a[0] * b[1] - a[1] * b[0];
+                if (k == 0) { n[0] = c0; n[1] = c1; n[2] = c2; }
+                else { n[0] = n[0] + c0; n[1] = n[1] + c1; n[2] = n[2] + c2; }
+            }
+            const float len = sqrtf((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
+            const float inv = len > 0.0f ? 1.0f / len : 0.0f;
+            out[w++] = org[0] + ((float)x + 0.5f) * cell;
+            out[w++] = y;
+            out[w++] = org[2] + ((float)z + 0.5f) * cell;
+            out[w++] = n[0] * inv;
+            out[w++] = n[1] * inv;
+            out[w++] = n[2] * inv;
+        }
+}
+
+/* Grid::genIndices (grid.h:118-136) for nx x nz vertices; returns the index
+ * count (idx may be NULL to count). */
+int64_t orc_terrain_indices(int nx, int nz, uint32_t* idx)
+{
+    int64_t k = 0;
+    for (int z = 0, j = nz - 1; z < nz && j >= 0; z++, j--)
+        for (int x = 0, i = nx - 1; x < nx && i >= 0; x++, i--) {
+            if (x + 1 < nx && z + 1 < nz) {
+                if (idx) {
+                    idx[k] = (uint32_t)(z * nx + x);
+                    idx[k + 1] = (uint32_t)(z * nx + x + 1);
+                    idx[k + 2] = (uint32_t)((z + 1) * nx + x);
+                }
+                k += 3;
+            }
+            if (j - 1 >= 0 && i - 1 >= 0) {
+                if (idx) {
+                    idx[k] = (uint32_t)(j * nx + i);
+                    idx[k + 1] = (uint32_t)(j * nx + i - 1);
+                    idx[k + 2] = (uint32_t)((j - 1) * nx + i);
+                }
+                k += 3;
+            }
+        }
+    return k;
 }
 
 int64_t orc_neighbours_brute(const float* pos3, int n, float thr, int32_t* counts,

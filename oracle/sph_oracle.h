@@ -159,6 +159,9 @@ int orc_step(const orc_scene* s, int real, int comp, int n,
 
 /* Terrain heights from fixed-point volumes: t_org[1] + (float)vol * hq. */
 void orc_heights(const orc_consts* c, int tcells, const int64_t* vol_fx, float* height);
+/* Terrain mesh in Grid::UpdateGrid / genIndices form (grid.h:118-176). */
+void orc_terrain_mesh(const float* H, int nx, int nz, float cell, const float* org, float* out);
+int64_t orc_terrain_indices(int nx, int nz, uint32_t* idx);
 
 /* Erosion / deposition exchange alone (the step's last stage) from a given
  * post-integrate state, in input order.  Returns the number of particles

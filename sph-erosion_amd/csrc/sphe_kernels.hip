@@ -1272,6 +1272,47 @@ __global__ __launch_bounds__(256) void k_keys(DevState D, int scene, int32_t* __
     keys[slot - S.p_begin] = (kz * S.dims[1] + ky) * S.dims[0] + kx;
 }
 
+// Grid::UpdateGrid's vertex stream (grid.h:138-176) of one scene's terrain,
+// one thread per cell: the cell centre (org + (i + 0.5) cell), the height,
+// and normalize(cross(u,l) + cross(u,r) + cross(d,l) + cross(d,r)) over the
+// vectors to the edge neighbours present (normalize = v * (1 / |v|), as
+// glm's; a zero sum gives a zero normal).
+__global__ __launch_bounds__(256) void k_terrain_mesh(const float* __restrict__ H, int nx, int nz, float cell,
+                                                      float ox, float oz, float* __restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nx * nz) return;
+    const int x = (int)(t % nx), z = (int)(t / nx);
+    const float y = H[t];
+    float u[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f}, r[3] = {0.f, 0.f, 0.f}, l[3] = {0.f, 0.f, 0.f};
+    if (x - 1 >= 0) { l[0] = cell; l[1] = y - H[t - 1]; }
+    if (x + 1 < nx) { r[0] = cell; r[1] = H[t + 1] - y; }
+    if (z - 1 >= 0) { u[1] = y - H[t - nx]; u[2] = cell; }
+    if (z + 1 < nz) { d[1] = H[t + nx] - y; d[2] = cell; }
+    const float* A[4] = {u, u, d, d};
+    const float* B[4] = {l, r, l, r};
+    float n[3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float* a = A[k];
+        const float* b = B[k];
+        const float c0 = a[1] * b[2] - a[2] * b[1];
+        const float c1 = a[2] * b[0] - a[0] * b[2];
+        const float c2 = a[0] * b[1] - a[1] * b[0];
+        if (k == 0) { n[0] = c0; n[1] = c1; n[2] = c2; }
+        else { n[0] = n[0] + c0; n[1] = n[1] + c1; n[2] = n[2] + c2; }
+    }
+    const float len = d_sqrt((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
+    const float inv = len > 0.f ? 1.f / len : 0.f;
+    float* o = out + 6 * t;
+    o[0] = ox + ((float)x + 0.5f) * cell;
+    o[1] = y;
+    o[2] = oz + ((float)z + 0.5f) * cell;
+    o[3] = n[0] * inv;
+    o[4] = n[1] * inv;
+    o[5] = n[2] * inv;
+}
+
 __global__ void k_clear_status(uint32_t* status)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *status = 0u;
@@ -1450,6 +1491,14 @@ void sphe_launch_two_phase(const DevState& D, hipStream_t st, hipEvent_t e0, hip
 }
 
 void sphe_launch_clear_status(const DevState& D, hipStream_t st) { k_clear_status<<<1, 64, 0, st>>>(D.status); }
+
+void sphe_launch_terrain_mesh(const float* H, int nx, int nz, float cell, float ox, float oz, float* out,
+                              hipStream_t st)
+{
+    const int64_t n = (int64_t)nx * nz;
+    if (n == 0) return;
+    k_terrain_mesh<<<grid_of(n, 256), 256, 0, st>>>(H, nx, nz, cell, ox, oz, out);
+}
 
 template <int G>
 static void launch_inspect_g(const DevState& D, int64_t slot, double* out, hipStream_t st)

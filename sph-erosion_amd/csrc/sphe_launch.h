@@ -20,6 +20,8 @@ void sphe_launch_force(const DevState& D, int block, int remap, int lanes, hipSt
 void sphe_launch_heights(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void sphe_launch_two_phase(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void sphe_launch_clear_status(const DevState& D, hipStream_t st);
+void sphe_launch_terrain_mesh(const float* H, int nx, int nz, float cell, float ox, float oz, float* out,
+                              hipStream_t st);
 void sphe_launch_inspect(const DevState& D, int64_t slot, int lanes, double* out, hipStream_t st);
 void sphe_launch_neighbours(const DevState& D, int scene, const int64_t* off, int32_t* counts, int32_t* idx,
                             hipStream_t st);

@@ -95,6 +95,8 @@ struct sphe_ctx {
     double* d_mass_part = nullptr;
     double* d_mass_out = nullptr;
     double* d_inspect = nullptr;
+    float* d_mesh = nullptr;  // terrain vertex stream (sphe_terrain_mesh*)
+    size_t mesh_cap = 0;      // floats
 
     // sphe_save_state / sphe_restore_state: particle and terrain state of
     // every scene, in the host staging layout
@@ -975,6 +977,7 @@ int sphe_destroy(sphe_ctx* c)
     }
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     free_device(c);
+    if (c->d_mesh) (void)hipFree(c->d_mesh);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SPHE_OK;
@@ -1376,6 +1379,29 @@ int sphe_terrain_heights(sphe_ctx* c, int scene, float* out, size_t count)
     return SPHE_OK;
 }
 
+// The scene's terrain vertex stream computed on the device (k_terrain_mesh)
+// into the context's mesh buffer, which grows as needed.
+int mesh_on_device(sphe_ctx* c, const Scene* s, float** out)
+{
+    const size_t nv = (size_t)s->tnx * s->tnz * 6;
+    int r = ensure_device(c);
+    if (r) return r;
+    if (nv > c->mesh_cap) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->d_mesh) (void)hipFree(c->d_mesh);
+        c->d_mesh = nullptr;
+        c->mesh_cap = 0;
+        HIPCHK(c, hipMalloc(&c->d_mesh, nv * 4));
+        c->mesh_cap = nv;
+    }
+    sphe_launch_terrain_mesh(c->D.height + s->t_off, s->tnx, s->tnz, s->t_cell, s->t_org[0], s->t_org[2],
+                             c->d_mesh, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = c->d_mesh;
+    return SPHE_OK;
+}
+
 int sphe_terrain_mesh(sphe_ctx* c, int scene, float* verts, size_t* nverts, uint32_t* indices, size_t* nindices)
 {
     SCENE_OR_FAIL(s, c, scene);
@@ -1394,67 +1420,48 @@ int sphe_terrain_mesh(sphe_ctx* c, int scene, float* verts, size_t* nverts, uint
     if (!verts && !indices) return SPHE_OK;
     if ((verts && (!nverts || *nverts < nv)) || (indices && (!nindices || *nindices < ni)))
         return fail(c, SPHE_EARG, "mesh output too small");
-    std::vector<float> H(nv / 6 + 1);
-    if (nx > 0 && nz > 0) {
-        int r = sphe_terrain_heights(c, scene, H.data(), H.size());
-        if (r) return r;
-    }
     if (verts) {
-        // UpdateGrid vertex stream (grid.h:138-176): position and the
-        // normalised sum of the four neighbour cross products
-        auto at = [&](int x, int z) { return H[(size_t)z * nx + x]; };
-        auto cross = [](const float* a, const float* b, float* o) {
-            o[0] = a[1] * b[2] - a[2] * b[1];
-            o[1] = a[2] * b[0] - a[0] * b[2];
-            o[2] = a[0] * b[1] - a[1] * b[0];
-        };
-        const float tc = s->t_cell;
-        size_t w = 0;
-        for (int z = 0; z < nz; ++z)
-            for (int x = 0; x < nx; ++x) {
-                const float y = at(x, z);
-                const float px = s->t_org[0] + ((float)x + 0.5f) * tc;
-                const float pz = s->t_org[2] + ((float)z + 0.5f) * tc;
-                float u[3] = {0, 0, 0}, d[3] = {0, 0, 0}, r3[3] = {0, 0, 0}, l[3] = {0, 0, 0};
-                if (x - 1 >= 0) { l[0] = tc; l[1] = y - at(x - 1, z); }
-                if (x + 1 < nx) { r3[0] = tc; r3[1] = at(x + 1, z) - y; }
-                if (z - 1 >= 0) { u[1] = y - at(x, z - 1); u[2] = tc; }
-                if (z + 1 < nz) { d[1] = at(x, z + 1) - y; d[2] = tc; }
-                float a1[3], a2[3], a3[3], a4[3];
-                cross(u, l, a1);
-                cross(u, r3, a2);
-                cross(d, l, a3);
-                cross(d, r3, a4);
-                float n3[3] = {a1[0] + a2[0] + a3[0] + a4[0], a1[1] + a2[1] + a3[1] + a4[1],
-                               a1[2] + a2[2] + a3[2] + a4[2]};
-                const float len = std::sqrt((n3[0] * n3[0] + n3[1] * n3[1]) + n3[2] * n3[2]);
-                const float inv = len > 0.0f ? 1.0f / len : 0.0f;
-                verts[w++] = px;
-                verts[w++] = y;
-                verts[w++] = pz;
-                verts[w++] = n3[0] * inv;
-                verts[w++] = n3[1] * inv;
-                verts[w++] = n3[2] * inv;
-            }
+        // UpdateGrid vertex stream (grid.h:138-176), from the device
+        if (nv > 0) {
+            float* d = nullptr;
+            int r = mesh_on_device(c, s, &d);
+            if (r) return r;
+            HIPCHK(c, hipMemcpy(verts, d, nv * 4, hipMemcpyDeviceToHost));
+        }
         *nverts = nv;
     }
     if (indices) {
-        size_t w = 0;
+        size_t k = 0;
         for (int z = 0, j = nz - 1; z < nz && j >= 0; ++z, --j)
             for (int x = 0, i = nx - 1; x < nx && i >= 0; ++x, --i) {
                 if (x + 1 < nx && z + 1 < nz) {
-                    indices[w++] = (uint32_t)(z * nx + x);
-                    indices[w++] = (uint32_t)(z * nx + (x + 1));
-                    indices[w++] = (uint32_t)((z + 1) * nx + x);
+                    indices[k++] = (uint32_t)(z * nx + x);
+                    indices[k++] = (uint32_t)(z * nx + x + 1);
+                    indices[k++] = (uint32_t)((z + 1) * nx + x);
                 }
                 if (j - 1 >= 0 && i - 1 >= 0) {
-                    indices[w++] = (uint32_t)(j * nx + i);
-                    indices[w++] = (uint32_t)(j * nx + (i - 1));
-                    indices[w++] = (uint32_t)((j - 1) * nx + i);
+                    indices[k++] = (uint32_t)(j * nx + i);
+                    indices[k++] = (uint32_t)(j * nx + i - 1);
+                    indices[k++] = (uint32_t)((j - 1) * nx + i);
                 }
             }
         *nindices = ni;
     }
+    return SPHE_OK;
+}
+
+int sphe_terrain_mesh_device(sphe_ctx* c, int scene, const float** dverts, size_t* nfloats)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (!dverts || !nfloats) return fail(c, SPHE_EARG, "null output");
+    const size_t nv = (size_t)s->tnx * s->tnz * 6;
+    *nfloats = nv;
+    *dverts = nullptr;
+    if (nv == 0) return SPHE_OK;
+    float* d = nullptr;
+    int r = mesh_on_device(c, s, &d);
+    if (r) return r;
+    *dverts = d;
     return SPHE_OK;
 }
 

@@ -16,6 +16,9 @@ that has the reference; the fixtures are committed, the reference is not.
                    g = (0,-9.8,0), dt = 1e-3: input and state after 1 step
   f4_box.npz       isolated particles crossing 1, 2 or 3 walls, on a wall, and
                    at rest outside the box: input and state after 1 step
+  mesh_ref.npz     the reference's Grid(24, 24, 24) UpdateGrid on a 512 x 512
+                   uint8 heightfield with smooth heights in [0, 22] in the
+                   24 x 24 corner: GetSurfaceParts and GetIndices
 """
 import os
 import sys
@@ -105,6 +108,16 @@ def main():
     np.savez_compressed(os.path.join(HERE, "f4_box.npz"), in_pos=P["Position"], in_vel=P["Velocity"],
                         Position=R["Position"], Velocity=R["Velocity"], Acceleration=R["Acceleration"],
                         Density=R["Density"], NeighbId=R["NeighbId"])
+    # terrain mesh (grid.h:118-176): smooth integer heights below the clamp
+    # of UpdateGrid (dimy - 1) in the grid's corner of the 512 x 512 image
+    d = 24
+    x = np.arange(d)
+    hx = 11.0 + 7.0 * np.sin(x / 3.1)[:, None] + 4.0 * np.cos(x / 2.3)[None, :]
+    h = np.clip(np.rint(hx), 0, d - 2).astype(np.uint8)  # h[x, z]
+    img = np.zeros((512, 512), dtype=np.uint8)
+    img[:d, :d] = h
+    v, i = O.refbin_grid_mesh(img, d)
+    np.savez_compressed(os.path.join(HERE, "mesh_ref.npz"), h_xz=h, verts=v, indices=i)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

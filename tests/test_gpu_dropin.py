@@ -7,10 +7,13 @@ import numpy as np
 import pytest
 
 import oracle as O
+import sphe
+from sphe import scenes
 
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
 
 
 def test_cpp_front_end_matches_reference(tmp_path):
@@ -39,3 +42,49 @@ def test_cpp_front_end_matches_reference(tmp_path):
     assert lines[2] == "n 1125"   # AddParticles(125) appends a 5^3 lattice
     assert lines[3] == "n 1000"   # Reset re-creates Initialize's count
     assert lines[4] == "range_error 1"
+
+
+def _hip_copy(dptr, nbytes):
+    """Device to host copy through the HIP runtime (the test reads the
+    library's device mesh buffer)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    out = np.zeros(nbytes // 4, dtype=np.float32)
+    rc = hip.hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(dptr), C.c_size_t(nbytes), C.c_int(2))
+    assert rc == 0
+    return out
+
+
+def test_terrain_mesh_is_the_reference_grid_mesh():
+    """In voxel units (cell 1, cell centres at integer x, z, integer heights)
+    the device-computed mesh equals the reference's Grid::UpdateGrid /
+    genIndices output bit for bit (fixture mesh_ref.npz)."""
+    g = np.load(os.path.join(GOLD, "mesh_ref.npz"))
+    h = g["h_xz"].astype(np.float32)
+    d = h.shape[0]
+    p = sphe.default_params(dt=0.0)
+    ctx = sphe.Context()
+    sid = ctx.add_scene(p, dict(nx=d, nz=d, origin=(-0.5, 0.0, -0.5), cell=1.0, heights=h.T.copy()))
+    assert np.array_equal(ctx.terrain_heights(sid), h.T)
+    verts, idx = ctx.terrain_mesh(sid)
+    assert np.array_equal(verts.reshape(-1).view(np.uint32), g["verts"].view(np.uint32))
+    assert np.array_equal(idx, g["indices"])
+
+
+def test_terrain_mesh_after_erosion_matches_oracle():
+    """After erosion steps the mesh of the eroded heights (host copy and the
+    device buffer) equals the oracle's restatement bit for bit."""
+    import ctypes as C
+    sc = scenes.cfg1(n=4096, erosion=sphe.EROSION_ATOMIC)
+    ctx = sphe.Context()
+    sid = ctx.add_synthetic(sc)
+    ctx.step(30)
+    H = ctx.terrain_heights(sid)
+    ref = O.terrain_mesh(H, sc.terrain["cell"], sc.terrain["origin"])
+    verts, idx = ctx.terrain_mesh(sid)
+    assert np.array_equal(verts.reshape(-1).view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(idx, O.terrain_indices(H.shape[1], H.shape[0]))
+    dptr, n = C.c_void_p(), C.c_size_t()
+    assert sphe.lib().sphe_terrain_mesh_device(ctx._p, sid, C.byref(dptr), C.byref(n)) == 0
+    assert n.value == ref.size and dptr.value
+    assert np.array_equal(_hip_copy(dptr.value, n.value * 4).view(np.uint32), ref.view(np.uint32))

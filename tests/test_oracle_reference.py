@@ -197,3 +197,16 @@ def test_neighbour_brute_force_matches_reference_predicate():
     assert np.array_equal(cnt, acc.sum(1))
     for i in range(0, 400, 37):
         assert np.array_equal(idx[off[i]:off[i + 1]], np.nonzero(acc[i])[0])
+
+
+def test_terrain_mesh_matches_reference_grid():
+    """The oracle's mesh of a float heightfield equals the reference's
+    Grid::UpdateGrid / genIndices (grid.h:118-176) bit for bit in voxel units
+    (cell 1, cell centres at integer x, z), fixture mesh_ref.npz."""
+    g = np.load(os.path.join(GOLD, "mesh_ref.npz"))
+    h = g["h_xz"].astype(np.float32)  # h[x, z] = GetHeightfieldAt(x, z)
+    d = h.shape[0]
+    v = O.terrain_mesh(h.T, 1.0, (-0.5, 0.0, -0.5))
+    assert v.shape == g["verts"].shape
+    assert np.array_equal(v.view(np.uint32), g["verts"].view(np.uint32))
+    assert np.array_equal(O.terrain_indices(d, d), g["indices"])
