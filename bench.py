@@ -47,6 +47,13 @@ def make_workload(name):
         return scenes.cfg3(), "256 batched dam-break scenes, N log-uniform [2048, 65536], T in {64..256}"
     if name == "cfg4":
         return [scenes.cfg4()], "layer of 4,194,304 particles over ridged 2048^2 terrain, dt=5e-4"
+    if name == "cfg5":
+        return scenes.cfg5(), ("1,024 batched dam-break scenes, N log-uniform [1024, 16384], plus one "
+                               "8,388,608-particle layer on a 4096^2 terrain, dt=5e-4")
+    if name == "cfg5_share":
+        sc = scenes.cfg5_share()
+        return [sc], (f"one GPU's x-slab of cfg5's 8,388,608-particle layer: {sc.n:,} particles "
+                      "(1/8 of the layer plus a one-cell ghost layer) on the 4096^2 terrain")
     raise SystemExit(f"unknown workload {name}")
 
 
