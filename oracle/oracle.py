@@ -95,6 +95,7 @@ def lib():
         L.orc_derive.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcConsts)]
         L.orc_pred_threshold.argtypes = [C.c_float]
         L.orc_pred_threshold.restype = C.c_float
+        L.orc_root_f32_n.argtypes = [P, C.c_int, P, P]
         L.orc_cell_keys.argtypes = [C.POINTER(OrcConsts), P, C.c_int, P]
         L.orc_neighbours_brute.argtypes = [P, C.c_int, C.c_float, P, P, P]
         L.orc_neighbours_brute.restype = C.c_int64
@@ -208,6 +209,15 @@ def derive(scene):
 
 def pred_threshold(h):
     return lib().orc_pred_threshold(float(np.float32(h)))
+
+
+def root_f32(d2):
+    """(r, 1/r) of FP32 squared distances as the FP32 kernels compute them."""
+    d2 = np.ascontiguousarray(d2, dtype=np.float32).ravel()
+    r = np.empty_like(d2)
+    inv = np.empty_like(d2)
+    lib().orc_root_f32_n(d2.ctypes.data, len(d2), r.ctypes.data, inv.ctypes.data)
+    return r, inv
 
 
 def cell_keys(consts, pos):

@@ -14,6 +14,10 @@
  *                      plus closed forms (isolated particle, lattice density).
  *   orc_pred_threshold fluid_system.h:118-119 + glm length.  Pinned by an
  *                      exhaustive ulp walk against sqrtf and brute force.
+ *   orc_root_f32       length and normalize of fluid_system.h:118, 438-441
+ *                      as the FP32 kernels evaluate them (Newton 1/sqrt).
+ *                      Pinned by ulp bounds against sqrtf and 1/sqrtf over
+ *                      the FP32 range, and d2 = 0 -> r = 0.
  *   orc_cell_keys /    build-defined grid search.  Pinned by set equality of
  *   orc_step neighbours the grid neighbour sets with orc_neighbours_brute.
  *   orc_step (FP64)    the reference's SPH terms; pinned against ref_step on
@@ -289,6 +293,33 @@ int ref_num_threads(void)
 /* orc: the build's step                                                   */
 /* ====================================================================== */
 
+/* r = |d| and 1/r from d2 = |d|^2 as the FP32 kernels compute them
+ * (build-defined): three Newton steps y <- y (1.5 - (d2/2) y^2) for 1/sqrt
+ * from the integer estimate 0x5f375a86 - bits(d2)/2, each step a multiply,
+ * a fused multiply-add and a multiply; r = d2 y.  Only IEEE operations, so
+ * it replays the GPU bit for bit.  Pinned against correctly rounded sqrtf
+ * and 1/sqrtf over the FP32 range of squared distances (ulp bounds). */
+void orc_root_f32(float d2, float* r, float* inv)
+{
+    int32_t bits;
+    memcpy(&bits, &d2, sizeof bits);
+    bits = 0x5f375a86 - (bits >> 1);
+    float y;
+    memcpy(&y, &bits, sizeof y);
+    const float hx = 0.5f * d2;
+    for (int s = 0; s < 3; s++) {
+        const float t = hx * y;
+        y = y * fmaf(-t, y, 1.5f);
+    }
+    *inv = y;
+    *r = d2 * y;
+}
+
+void orc_root_f32_n(const float* d2, int n, float* r, float* inv)
+{
+    for (int i = 0; i < n; i++) orc_root_f32(d2[i], &r[i], &inv[i]);
+}
+
 float orc_pred_threshold(float h)
 {
     /* glm::length(r) <= h  <=>  sqrtf(d2) <= h  <=>  d2 <= T, with T the
@@ -501,8 +532,11 @@ static int cmp_sortrec(const void* a, const void* b)
 }
 
 /* ---- templated body: double oracle and float twin ---- */
+/* candidates per lane per iteration of the GPU density pass */
+#define ORC_DENSITY_UNROLL 4
 #define OT double
 #define OSFX _d
+#define OT_ROOT(d2, r, inv) do { (r) = sqrt(d2); (inv) = 1.0 / (r); } while (0)
 #define OT_SQRT sqrt
 #define OT_FLOOR floor
 #define OT_FABS fabs
@@ -510,6 +544,7 @@ static int cmp_sortrec(const void* a, const void* b)
 #include "orc_body.inc"
 #undef OT
 #undef OSFX
+#undef OT_ROOT
 #undef OT_SQRT
 #undef OT_FLOOR
 #undef OT_FABS
@@ -517,6 +552,7 @@ static int cmp_sortrec(const void* a, const void* b)
 
 #define OT float
 #define OSFX _f
+#define OT_ROOT(d2, r, inv) orc_root_f32((d2), &(r), &(inv))
 #define OT_SQRT sqrtf
 #define OT_FLOOR floorf
 #define OT_FABS fabsf
@@ -524,6 +560,7 @@ static int cmp_sortrec(const void* a, const void* b)
 #include "orc_body.inc"
 #undef OT
 #undef OSFX
+#undef OT_ROOT
 #undef OT_SQRT
 #undef OT_FLOOR
 #undef OT_FABS

@@ -127,6 +127,11 @@ void orc_derive(const orc_scene* s, orc_consts* c);
  * predicate glm::length(r) <= h (fluid_system.h:118-119). */
 float orc_pred_threshold(float h);
 
+/* r = |d| and 1/r from d2 = |d|^2 as the FP32 kernels compute them: three
+ * Newton steps for 1/sqrt from an integer estimate, r = d2 / sqrt(d2). */
+void orc_root_f32(float d2, float* r, float* inv);
+void orc_root_f32_n(const float* d2, int n, float* r, float* inv);
+
 /* Cell key (x fastest) of each position, FP32. */
 void orc_cell_keys(const orc_consts* c, const float* pos3, int n, int32_t* keys);
 

@@ -634,19 +634,23 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
                 const T dx = pi.x - p.x, dy = pi.y - p.y, dz = pi.z - p.z;
                 d2[u] = (dx * dx + dy * dy) + dz * dz;
             }
+            // The lane's U terms (0 for a candidate outside the radius or
+            // past the stream) are added in plain arithmetic, u = 0 first,
+            // and the block's partial enters the lane's sum.
             // byte u of x: 1 if this lane lists its u-th candidate
             uint32_t x = 0;
+            T part = T(0);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const bool in = ok[u] & (d2[u] <= thr);
-                if (in) {
-                    T q = h2 - d2[u];
-                    q = q > T(0) ? q : T(0);
-                    sum.add((q * q) * q);
-                }
+                T q = h2 - d2[u];
+                q = q > T(0) ? q : T(0);
+                const T w = in ? (q * q) * q : T(0);
+                part = u ? part + w : w;
                 // the force pass adds the particle's own term first: self is not listed
                 x |= (uint32_t)(in & (jj[u] != self)) << (8 * u);
             }
+            sum.add(part);
             // canonical place of the lane's u-th candidate (position u*G + l
             // of the block): the block's listed candidates at u' < u (all
             // lanes), then those of lanes below l at u.  Byte-wise sums over
@@ -711,6 +715,37 @@ struct ForceOut {
 // The terms one neighbour j != i adds to the ten force sums: pressure,
 // viscosity, colour normal and colour Laplacian (fluid_system.h:136-150,
 // 165-172) with the constants factored out.
+// r = |d| and 1/r from d2 = |d|^2 (fluid_system.h:118, 438-441: length and
+// normalize).  FP64: correctly rounded sqrt and division.  FP32: 1/sqrt by
+// three Newton steps y <- y (1.5 - (d2/2) y^2) from the integer estimate
+// 0x5f375a86 - bits(d2)/2 (relative error 3.4e-2 -> 1.8e-3 -> 4.6e-6 ->
+// FP32 rounding), each step one multiply, one fused multiply-add and one
+// multiply; r = d2 / sqrt(d2) = d2 y.  Within a few ulp of the correctly
+// rounded values and made only of IEEE operations, so a CPU replays it bit
+// for bit; it replaces two correctly rounded sequences of about ten
+// instructions each.  d2 = 0 gives r = 0 (the coincident-pair branch).
+template <typename T> struct Root {
+    static __device__ __forceinline__ void of(T d2, T& r, T& inv)
+    {
+        r = d_sqrt(d2);
+        inv = T(1) / r;
+    }
+};
+template <> struct Root<float> {
+    static __device__ __forceinline__ void of(float d2, float& r, float& inv)
+    {
+        float y = __int_as_float(0x5f375a86 - (__float_as_int(d2) >> 1));
+        const float hx = 0.5f * d2;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            const float t = hx * y;
+            y = y * __builtin_fmaf(-t, y, 1.5f);
+        }
+        inv = y;
+        r = d2 * y;
+    }
+};
+
 template <typename T> struct Terms {
     T lap, fp0, fp1, fp2, fv0, fv1, fv2, nn0, nn1, nn2;
 };
@@ -723,12 +758,12 @@ __device__ __forceinline__ Terms<T> pair_terms(const SceneConsts<T>& K, T dx, T 
     T q = K.h2 - d2;
     q = q > T(0) ? q : T(0);
     t.lap = b_j * (q * (K.t3h2 - T(7) * d2));
-    const T r = d_sqrt(d2);
+    T r, inv;
+    Root<T>::of(d2, r, inv);
     const T hr = K.h - r;
     // coincident pair: the reference points along normalize(1,1,1)
-    // (fluid_system.h:438-440); the quotient is discarded then
+    // (fluid_system.h:438-440); the reciprocal is discarded then
     const bool near = r < K.eps_r;
-    const T inv = T(1) / r;
     const T ux = near ? K.inv_sqrt3 : dx * inv;
     const T uy = near ? K.inv_sqrt3 : dy * inv;
     const T uz = near ? K.inv_sqrt3 : dz * inv;

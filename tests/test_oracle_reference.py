@@ -186,6 +186,37 @@ def test_predicate_threshold_exhaustive_walk():
     assert np.float32(O.pred_threshold(np.float32(0.0457))).view(np.uint32) == 0x3B08DF0C
 
 
+def test_fp32_root_is_within_ulps_of_sqrt():
+    """The FP32 kernels' r = |d| and 1/r (three Newton steps for 1/sqrt from
+    an integer estimate) against correctly rounded sqrt and 1/sqrt: within
+    2.5 and 2 ulp wherever d2 / 2 is a normal float (two Newton steps would
+    be ~80 ulp off), r = 0 for d2 = 0, and below the coincident-pair radius
+    for smaller d2 (there 1/r is discarded and h - r rounds to h)."""
+    rng = np.random.default_rng(11)
+    d2 = np.concatenate([
+        np.exp2(rng.uniform(-124, 127, 400_000)),
+        np.arange(1, 3000, dtype=np.float64) ** 2,
+        np.exp2(np.arange(-124, 127, dtype=np.float64)),
+        [np.nextafter(np.float32(1), np.float32(0)), np.finfo(np.float32).max],
+    ]).astype(np.float32)
+    r, inv = O.root_f32(d2)
+    ex = np.sqrt(d2.astype(np.float64))
+
+    def ulps(a, e):
+        return np.abs(a.astype(np.float64) - e) / np.spacing(np.abs(e).astype(np.float32)).astype(np.float64)
+
+    assert ulps(r, ex).max() <= 2.5
+    assert ulps(inv, 1.0 / ex).max() <= 2.0
+    r0, _ = O.root_f32(np.zeros(1, np.float32))
+    assert r0[0] == 0.0
+    sub = np.exp2(rng.uniform(-149, -124, 10_000)).astype(np.float32)
+    rs, ins = O.root_f32(sub)
+    assert np.isfinite(rs).all() and np.isfinite(ins).all()
+    assert (rs < np.nextafter(np.float32(1e-4), np.float32(1))).all()
+    h = np.float32(0.0457)
+    assert (np.float32(h) - rs == h).all()
+
+
 def test_neighbour_brute_force_matches_reference_predicate():
     rng = np.random.default_rng(3)
     pos = rng.uniform(-0.1, 0.1, size=(400, 3)).astype(np.float32)
