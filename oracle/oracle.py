@@ -48,7 +48,7 @@ class OrcScene(C.Structure):
                 ("tnx", C.c_int32), ("tnz", C.c_int32), ("t_org", C.c_double * 3),
                 ("t_cell", C.c_double), ("fx_exp", C.c_int32), ("erosion", C.c_int32),
                 ("kc", C.c_double), ("ke", C.c_double), ("kd", C.c_double), ("band", C.c_double),
-                ("lanes", C.c_int32), ("slabs", C.c_int32)]
+                ("lanes", C.c_int32), ("slabs", C.c_int32), ("yz", C.c_int32)]
 
 
 class OrcConsts(C.Structure):
@@ -61,7 +61,8 @@ class OrcConsts(C.Structure):
         ("t_org", C.c_float * 3), ("t_inv", C.c_float), ("hq", C.c_float),
         ("kcv", C.c_float), ("ke", C.c_float), ("kd", C.c_float), ("band", C.c_float),
         ("up", C.c_float), ("dn", C.c_float), ("dmax", C.c_float),
-        ("dh", C.c_double), ("dh2", C.c_double), ("dthr_unused", C.c_double)]
+        ("dh", C.c_double), ("dh2", C.c_double), ("dthr_unused", C.c_double),
+        ("inv_yz", C.c_float), ("yz_scale", C.c_float), ("yz", C.c_int32)]
 
 
 class OrcDiag(C.Structure):
@@ -184,12 +185,16 @@ def refbin_step(particles, params, steps=1):
 
 # ------------------------------------------------------------------ orc
 
+# cell split along y and z of the product library's default grid
+YZ_DEFAULT = 2
+
+
 def make_scene(**kw):
     s = OrcScene()
     defaults = dict(h=0.0457, mass=0.02, rho0=998.29, k=3.0, visc=3.5, surf_tens=0.0728,
                     g=(0.0, -9.82, 0.0), dt=0.0, box_c=(0.0, 0.0, 0.0), box_e=0.2,
                     tnx=0, tnz=0, t_org=(0.0, 0.0, 0.0), t_cell=1.0, fx_exp=60, erosion=0,
-                    kc=0.05, ke=0.3, kd=0.3, band=0.0457, lanes=4, slabs=4)
+                    kc=0.05, ke=0.3, kd=0.3, band=0.0457, lanes=4, slabs=4, yz=YZ_DEFAULT)
     defaults.update(kw)
     for k, v in defaults.items():
         if isinstance(v, (tuple, list, np.ndarray)):

@@ -364,11 +364,15 @@ void orc_derive(const orc_scene* s, orc_consts* c)
         c->box_lo[a] = c->box_c[a] - c->box_e;
         c->box_hi[a] = c->box_c[a] + c->box_e;
     }
+    /* cells: h (1 + 2^-8) along x, 1/yz of that along y and z (build-defined) */
+    c->yz = s->yz == 2 ? 2 : 1;
     c->cell = h * 1.00390625f; /* h (1 + 2^-8) */
     c->inv_cell = 1.0f / c->cell;
+    c->inv_yz = c->inv_cell * (float)c->yz; /* exact */
+    c->yz_scale = 1.0f / (float)c->yz;
     for (int a = 0; a < 3; a++) {
         c->grid_org[a] = c->box_lo[a];
-        float ext = (c->box_hi[a] - c->box_lo[a]) * c->inv_cell;
+        float ext = (c->box_hi[a] - c->box_lo[a]) * (a == 0 ? c->inv_cell : c->inv_yz);
         c->dims[a] = (int)floorf(ext) + 1;
     }
     c->dh = hd;
@@ -409,7 +413,7 @@ void orc_cell_keys(const orc_consts* c, const float* pos3, int n, int32_t* keys)
     for (int i = 0; i < n; i++) {
         int k[3];
         for (int a = 0; a < 3; a++) {
-            float t = (pos3[3 * i + a] - c->grid_org[a]) * c->inv_cell;
+            float t = (pos3[3 * i + a] - c->grid_org[a]) * (a == 0 ? c->inv_cell : c->inv_yz);
             k[a] = clamp_index_f(floorf(t), c->dims[a] - 1);
         }
         keys[i] = (k[2] * c->dims[1] + k[1]) * c->dims[0] + k[0];

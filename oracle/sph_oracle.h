@@ -102,6 +102,7 @@ typedef struct {
     double kc, ke, kd, band;        /* capacity coefficient, erode/deposit rates, contact band */
     int32_t lanes;                  /* lanes per particle splitting the neighbour sums (1, 2, 4, 8) */
     int32_t slabs;                  /* x slabs per grid cell (1, 2, 4): order and row clipping */
+    int32_t yz;                     /* cells are 1/yz of the x edge along y and z (1 or 2; 0 = 1) */
 } orc_scene;
 
 /* Derived per-scene constants, identical in meaning to the GPU's. */
@@ -119,6 +120,9 @@ typedef struct {
     float t_org[3], t_inv, hq;  /* height = t_org[1] + (float)vol_fx * hq */
     float kcv, ke, kd, band, up, dn, dmax;
     double dh, dh2, dthr_unused; /* FP64 twins of h, h*h */
+    float inv_yz;   /* inv_cell * yz: y and z cell coordinates */
+    float yz_scale; /* 1 / yz: y/z cell units to x-cell units */
+    int32_t yz;
 } orc_consts;
 
 void orc_derive(const orc_scene* s, orc_consts* c);

@@ -26,6 +26,8 @@ struct SceneConsts {
     T rho0, k, mass, surf_tens, g[3], dt;
     T box_c[3], box_lo[3], box_hi[3];
     T inv_cell, org[3];
+    T inv_yz;    // inv_cell * yz: y and z cell coordinates
+    T yz_scale;  // 1 / yz: y/z cell units to x-cell units
     T t_org[3], t_inv, kcv, ke, kd, band, up, dn, dmax;
 };
 
@@ -74,6 +76,9 @@ struct DevState {
     // occupied cells (an empty cell's word may be stale: readers clamp the
     // offsets to the cell)
     uint32_t* subw; int32_t sub_log2;
+    // cells are 1/yz of the x edge along y and z (yz = 1 or 2): the stencil
+    // has (2 yz + 1)^2 rows
+    int32_t yz;
     int64_t* vol; float* height;
     int64_t* dq; int32_t* tcell;        // two-phase erosion records
     uint32_t* tcnt; uint32_t* tstart; uint32_t* tpartial; int64_t ntpartial;

@@ -89,8 +89,8 @@ __device__ __forceinline__ void cell_of(const SceneConsts<T>& K, const int32_t* 
                                         int& kx, int& ky, int& kz)
 {
     kx = cidx<T>(d_floor((x - K.org[0]) * K.inv_cell), dims[0] - 1);
-    ky = cidx<T>(d_floor((y - K.org[1]) * K.inv_cell), dims[1] - 1);
-    kz = cidx<T>(d_floor((z - K.org[2]) * K.inv_cell), dims[2] - 1);
+    ky = cidx<T>(d_floor((y - K.org[1]) * K.inv_yz), dims[1] - 1);
+    kz = cidx<T>(d_floor((z - K.org[2]) * K.inv_yz), dims[2] - 1);
 }
 
 // Neighbour-sum accumulators.  A particle's sum is split over the G lanes of
@@ -165,19 +165,38 @@ __device__ __forceinline__ T row_gap(T f)
 }
 
 // The stencil rows of a particle, clipped to what the smoothing sphere can
-// reach.  Rows are numbered r = 3 (dz + 1) + (dy + 1) (dz outer, dy inner).
-// In cell units the radius is h / cell < 1; a row whose y and z gaps g
-// satisfy gy^2 + gz^2 > 1 is skipped, and the others keep the x sub-cells
-// that meet [gx - w, gx + w], w = sqrt(1 - gy^2 - gz^2) + margin.  Every
-// neighbour is inside the range: the bounds are monotone in the position
-// and include a margin far above the rounding of the cell coordinates.
-// Cells with more than 1023 particles carry no slab offsets and are taken
-// whole.
+// reach.  Cells are h (1 + 2^-8) along x and 1/yz of that along y and z
+// (yz = 1 or 2), so the rows are the (2 yz + 1)^2 (y, z) columns around the
+// particle, numbered r = W (dz + yz) + (dy + yz), W = 2 yz + 1 (dz outer, dy
+// inner).  In x-cell units the radius is h / cell < 1; a row whose y and z
+// gaps g satisfy gy^2 + gz^2 > 1 is skipped, and the others keep the x
+// sub-cells that meet [gx - w, gx + w], w = sqrt(1 - gy^2 - gz^2) + margin.
+// Every neighbour is inside the range: the bounds are monotone in the
+// position and include a margin far above the rounding of the cell
+// coordinates.  Cells with more than 1023 particles carry no slab offsets
+// and are taken whole.
 template <typename T>
 struct RowProbe {
-    T gx, ylo, yhi, zlo, zhi, scale;
+    T gx, scale;
+    // gaps, in x-cell units, to the rows one and two cells below (l1, l2) and
+    // above (h1, h2) along y and z
+    T yl1, yl2, yh1, yh2, zl1, zl2, zh1, zh2;
     int kx, ky, kz, sl, smax, fmax, flo_min, fhi_max;
 };
+
+// r / W for 0 <= r < W^2 and W = 3 or 5 (multiply and shift)
+__host__ __device__ constexpr int row_div(int r, int W)
+{
+    return W == 3 ? (r * 11) >> 5 : (r * 13) >> 6;
+}
+constexpr bool row_div_exact()
+{
+    for (int W = 3; W <= 5; W += 2)
+        for (int r = 0; r < W * W; ++r)
+            if (row_div(r, W) != r / W) return false;
+    return true;
+}
+static_assert(row_div_exact(), "row_div");
 
 template <typename T>
 __device__ __forceinline__ RowProbe<T> row_probe(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
@@ -186,16 +205,21 @@ __device__ __forceinline__ RowProbe<T> row_probe(const DevState& D, const SceneD
     RowProbe<T> P;
     const int nx = S.dims[0], ny = S.dims[1], nz = S.dims[2];
     P.gx = (xi - K.org[0]) * K.inv_cell;
-    const T gy = (yi - K.org[1]) * K.inv_cell;
-    const T gz = (zi - K.org[2]) * K.inv_cell;
+    const T gy = (yi - K.org[1]) * K.inv_yz;
+    const T gz = (zi - K.org[2]) * K.inv_yz;
     P.kx = cidx<T>(d_floor(P.gx), nx - 1);
     P.ky = cidx<T>(d_floor(gy), ny - 1);
     P.kz = cidx<T>(d_floor(gz), nz - 1);
     const T fy = gy - T(P.ky), fz = gz - T(P.kz);
-    P.ylo = row_gap<T>(fy);
-    P.yhi = row_gap<T>(T(1) - fy);
-    P.zlo = row_gap<T>(fz);
-    P.zhi = row_gap<T>(T(1) - fz);
+    const T q = K.yz_scale;  // 1 / yz, exact
+    P.yl1 = row_gap<T>(fy * q);
+    P.yh1 = row_gap<T>((T(1) - fy) * q);
+    P.yl2 = row_gap<T>((T(1) + fy) * q);
+    P.yh2 = row_gap<T>((T(1) + (T(1) - fy)) * q);
+    P.zl1 = row_gap<T>(fz * q);
+    P.zh1 = row_gap<T>((T(1) - fz) * q);
+    P.zl2 = row_gap<T>((T(1) + fz) * q);
+    P.zh2 = row_gap<T>((T(1) + (T(1) - fz)) * q);
     P.sl = D.sub_log2;
     P.smax = (1 << P.sl) - 1;
     P.scale = T(1 << P.sl);
@@ -205,18 +229,23 @@ __device__ __forceinline__ RowProbe<T> row_probe(const DevState& D, const SceneD
     return P;
 }
 
-// Slot range [j0, j1) of row r; false when the row is outside the grid or
-// out of the sphere's reach.
 template <typename T>
-__device__ __forceinline__ bool row_range(const DevState& D, const SceneDev& S, const RowProbe<T>& P, int r,
-                                          uint32_t& j0, uint32_t& j1)
+__device__ __forceinline__ T gap_of(int d, T l1, T l2, T h1, T h2)
+{
+    return d == 0 ? T(0) : (d == -1 ? l1 : (d == 1 ? h1 : (d < 0 ? l2 : h2)));
+}
+
+// Slot range [j0, j1) of the row at (dy, dz); false when the row is outside
+// the grid or out of the sphere's reach.
+template <typename T>
+__device__ __forceinline__ bool row_range(const DevState& D, const SceneDev& S, const RowProbe<T>& P, int dyc,
+                                          int dzc, uint32_t& j0, uint32_t& j1)
 {
     const int nx = S.dims[0], ny = S.dims[1], nz = S.dims[2];
-    const int dzc = r / 3 - 1, dyc = r % 3 - 1;
     const int z = P.kz + dzc, y = P.ky + dyc;
     if (z < 0 || z >= nz || y < 0 || y >= ny) return false;
-    const T mz = dzc < 0 ? P.zlo : (dzc > 0 ? P.zhi : T(0));
-    const T my = dyc < 0 ? P.ylo : (dyc > 0 ? P.yhi : T(0));
+    const T mz = gap_of<T>(dzc, P.zl1, P.zl2, P.zh1, P.zh2);
+    const T my = gap_of<T>(dyc, P.yl1, P.yl2, P.yh1, P.yh2);
     const T r2 = my * my + mz * mz;
     if (r2 > T(1)) return false;
     const T w = d_sqrt(T(1) - r2) + T(SPHE_ROW_MARGIN);
@@ -258,10 +287,12 @@ __device__ __forceinline__ void for_each_row(const DevState& D, const SceneDev& 
                                              T xi, T yi, T zi, F&& row)
 {
     const RowProbe<T> P = row_probe<T>(D, S, K, xi, yi, zi);
-    for (int r = 0; r < 9; ++r) {
-        uint32_t j0, j1;
-        if (row_range<T>(D, S, P, r, j0, j1)) row(j0, j1);
-    }
+    const int yz = D.yz;
+    for (int dz = -yz; dz <= yz; ++dz)
+        for (int dy = -yz; dy <= yz; ++dy) {
+            uint32_t j0, j1;
+            if (row_range<T>(D, S, P, dy, dz, j0, j1)) row(j0, j1);
+        }
 }
 
 // Visits the candidates of the clipped stencil rows in ascending slot order
@@ -299,8 +330,8 @@ __device__ __forceinline__ void count_position(const DevState& D, const SceneDev
 {
     const T gx = (x - K.org[0]) * K.inv_cell;
     const int kx = cidx<T>(d_floor(gx), S.dims[0] - 1);
-    const int ky = cidx<T>(d_floor((y - K.org[1]) * K.inv_cell), S.dims[1] - 1);
-    const int kz = cidx<T>(d_floor((z - K.org[2]) * K.inv_cell), S.dims[2] - 1);
+    const int ky = cidx<T>(d_floor((y - K.org[1]) * K.inv_yz), S.dims[1] - 1);
+    const int kz = cidx<T>(d_floor((z - K.org[2]) * K.inv_yz), S.dims[2] - 1);
     const int64_t key = S.cell_off + ((int64_t)kz * S.dims[1] + ky) * S.dims[0] + kx;
     const int sub = sub_cell<T>(gx, kx, S.dims[0], D.sub_log2);
     D.key[slot] = (int32_t)((key << D.sub_log2) | sub);
@@ -577,15 +608,19 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     uint32_t stream_total = 0;
     // The particle's clipped rows as one stream of candidates, t = 0 ..
     // total-1 in canonical order.  Lane l computes rows l, l+G, ... and
-    // writes (first slot, length) to the group's row table in LDS.
-    __shared__ uint2 rowtab[512 / G * 9];
-    uint2* __restrict__ tab = rowtab + (threadIdx.x / G) * 9;
+    // writes (first slot, length) to the group's row table in LDS (dynamic:
+    // blockDim / G tables of W^2 entries).
+    const int yz = D.yz, W = 2 * yz + 1, nrows = W * W;
+    extern __shared__ uint2 rowtab[];
+    uint2* __restrict__ tab = rowtab + (threadIdx.x / G) * nrows;
     {
         const RowProbe<T> P = row_probe<T>(D, S, K, pi.x, pi.y, pi.z);
         uint32_t mine = 0;
-        for (int r = l; r < 9; r += G) {
+        for (int r = l; r < nrows; r += G) {
+            const int dz = row_div(r, W);
+            const int dy = r - dz * W;
             uint32_t j0 = 0, j1 = 0;
-            const bool on = row_range<T>(D, S, P, r, j0, j1);
+            const bool on = row_range<T>(D, S, P, dy - yz, dz - yz, j0, j1);
             const uint32_t len = on && j1 > j0 ? j1 - j0 : 0u;
             tab[r] = make_uint2(j0, len);
             mine += len;
@@ -616,7 +651,7 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
                 ok[u] = t < total;
                 jj[u] = self;
                 if (ok[u]) {
-                    while (t >= cur_end && cur < 8) {
+                    while (t >= cur_end && cur < nrows - 1) {
                         const uint2 e = tab[++cur];
                         cur_off = e.x - cur_end;
                         cur_end += e.y;
@@ -1435,6 +1470,15 @@ static void klaunch(void (*kern)(P...), dim3 grid, dim3 block, hipStream_t st, h
     else hipLaunchKernelGGL(kern, grid, block, 0, st, args...);
 }
 
+// The same with lds bytes of dynamic LDS per block.
+template <typename... P, typename... A>
+static void klaunch_lds(void (*kern)(P...), dim3 grid, dim3 block, size_t lds, hipStream_t st, hipEvent_t e0,
+                        hipEvent_t e1, A... args)
+{
+    if (e0 || e1) hipExtLaunchKernelGGL(kern, grid, block, lds, st, e0, e1, 0, args...);
+    else hipLaunchKernelGGL(kern, grid, block, lds, st, args...);
+}
+
 void sphe_launch_hash(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     if (D.P == 0) return;
@@ -1469,11 +1513,16 @@ void sphe_launch_reorder(const DevState& D, hipStream_t st, hipEvent_t e0, hipEv
 template <int G>
 static void launch_density_g(const DevState& D, int block, int remap, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
+    // one row table of W^2 (first slot, length) entries per group, at most
+    // 64 KiB per block (halve the block until it fits)
+    const int W = 2 * D.yz + 1;
+    while (block > 64 && (size_t)(block / G) * W * W * sizeof(uint2) > 65536) block /= 2;
+    const size_t lds = (size_t)(block / G) * W * W * sizeof(uint2);
     const int nblk = (int)grid_of(D.P * G, block);
     SPHE_PREC_SWITCH(D.precision,
-                     (klaunch(k_density<float, true, G>, nblk, block, st, e0, e1, D, nblk, remap)),
-                     (klaunch(k_density<float, false, G>, nblk, block, st, e0, e1, D, nblk, remap)),
-                     (klaunch(k_density<double, false, G>, nblk, block, st, e0, e1, D, nblk, remap)));
+                     (klaunch_lds(k_density<float, true, G>, nblk, block, lds, st, e0, e1, D, nblk, remap)),
+                     (klaunch_lds(k_density<float, false, G>, nblk, block, lds, st, e0, e1, D, nblk, remap)),
+                     (klaunch_lds(k_density<double, false, G>, nblk, block, lds, st, e0, e1, D, nblk, remap)));
 }
 
 template <int G>

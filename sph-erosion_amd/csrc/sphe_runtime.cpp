@@ -91,6 +91,7 @@ struct sphe_ctx {
     int maxn = 256;  // neighbour-list capacity per particle (grown when a step overflows it)
     int lanes = 4;   // lanes per particle in the density and force passes
     int sub_log2 = 2;  // x slabs per cell: 2^sub_log2
+    int yz = 2;        // cells are 1/yz of the x edge along y and z
 
     double* d_mass_part = nullptr;
     double* d_mass_out = nullptr;
@@ -153,19 +154,22 @@ float pred_threshold(float h)
     return t;
 }
 
-void grid_dims(const sphe_params& p, int dims[3])
+// Cells are cell = h (1 + 2^-8) along x (so 3 cells always cover the
+// radius under FP32 rounding) and cell / yz along y and z (2 yz + 1 cells).
+void grid_dims(const sphe_params& p, int dims[3], int yz)
 {
     const float h = p.h;
-    const float cell = h * 1.00390625f;  // h (1 + 2^-8): 27 cells always cover the radius
+    const float cell = h * 1.00390625f;
     const float inv = 1.0f / cell;
     for (int a = 0; a < 3; ++a) {
         const float lo = p.box_center[a] - p.box_half;
         const float hi = p.box_center[a] + p.box_half;
-        dims[a] = (int)std::floor((hi - lo) * inv) + 1;
+        const float ia = a == 0 ? inv : inv * (float)yz;  // exact: yz is 1 or 2
+        dims[a] = (int)std::floor((hi - lo) * ia) + 1;
     }
 }
 
-void derive(const Scene& s, SceneDev& d)
+void derive(const Scene& s, SceneDev& d, int yz)
 {
     const sphe_params& p = s.prm;
     SceneConsts<float>& f = d.f;
@@ -193,6 +197,8 @@ void derive(const Scene& s, SceneDev& d)
     f.dt = p.dt;
     const float cell = h * 1.00390625f;
     f.inv_cell = 1.0f / cell;
+    f.inv_yz = f.inv_cell * (float)yz;
+    f.yz_scale = 1.0f / (float)yz;
     for (int a = 0; a < 3; ++a) {
         f.g[a] = p.gravity[a];
         f.box_c[a] = p.box_center[a];
@@ -201,7 +207,7 @@ void derive(const Scene& s, SceneDev& d)
         f.org[a] = f.box_lo[a];
         f.t_org[a] = s.t_org[a];
     }
-    grid_dims(p, d.dims);
+    grid_dims(p, d.dims, yz);
     f.t_inv = 1.0f / s.t_cell;
     const double area = (double)s.t_cell * (double)s.t_cell;
     const double kc = (double)p.kc, rho0 = (double)p.rest_density;
@@ -231,6 +237,8 @@ void derive(const Scene& s, SceneDev& d)
     g.surf_tens = (double)p.surf_tens;
     g.dt = (double)p.dt;
     g.inv_cell = 1.0 / (hd * (1.0 + std::ldexp(1.0, -8)));
+    g.inv_yz = g.inv_cell * (double)yz;
+    g.yz_scale = 1.0 / (double)yz;
     for (int a = 0; a < 3; ++a) {
         g.g[a] = (double)p.gravity[a];
         g.box_c[a] = (double)f.box_c[a];
@@ -359,7 +367,7 @@ int build_layout(sphe_ctx* c)
         s.p_begin = P;
         s.p_cap = round_up((int64_t)s.ids.size(), SPHE_WAVE);
         P += s.p_cap;
-        grid_dims(s.prm, s.dims);
+        grid_dims(s.prm, s.dims, c->yz);
         s.ncell = (int64_t)s.dims[0] * s.dims[1] * s.dims[2];
         s.cell_off = C;
         C += s.ncell + 1;
@@ -396,6 +404,7 @@ int build_layout(sphe_ctx* c)
     DALLOC(D.cnt, (size_t)(C + 1) * 4);
     DALLOC(D.start, (size_t)(C + 1) * 4);
     DALLOC(D.subw, (size_t)(C + 1) * 4);
+    D.yz = c->yz;
     D.sub_log2 = c->sub_log2;
     while (D.sub_log2 > 0 && (C << D.sub_log2) > INT32_MAX) --D.sub_log2;  // keys are int32
     D.npartial = sphe_scan_partials(C + 1);
@@ -593,7 +602,7 @@ int upload_table(sphe_ctx* c, bool force)
     std::vector<SceneDev> t(c->scenes.size());
     for (size_t i = 0; i < c->scenes.size(); ++i) {
         std::memset(&t[i], 0, sizeof(SceneDev));
-        derive(c->scenes[i], t[i]);
+        derive(c->scenes[i], t[i], c->yz);
     }
     if (!force && c->table_uploaded && t.size() == c->table.size() &&
         std::memcmp(t.data(), c->table.data(), t.size() * sizeof(SceneDev)) == 0)
@@ -612,7 +621,7 @@ int check_grid(sphe_ctx* c)
 {
     for (Scene& s : c->scenes) {
         int dims[3];
-        grid_dims(s.prm, dims);
+        grid_dims(s.prm, dims, c->yz);
         if (dims[0] != s.dims[0] || dims[1] != s.dims[1] || dims[2] != s.dims[2]) {
             int r = begin_host_edit(c);
             if (r) return r;
@@ -990,7 +999,7 @@ int sphe_add_scene(sphe_ctx* c, const sphe_params* p, const sphe_terrain_desc* t
     if (!c || !p) return fail(c, SPHE_EARG, "null argument");
     if (!(p->h > 0.0f) || !(p->box_half > 0.0f)) return fail(c, SPHE_EARG, "h and box_half must be > 0");
     int dims[3];
-    grid_dims(*p, dims);
+    grid_dims(*p, dims, c->yz);
     if ((double)dims[0] * dims[1] * dims[2] > 4.0e8) return fail(c, SPHE_ERANGE, "grid too fine for the box");
     int r = begin_host_edit(c);
     if (r) return r;
@@ -1563,7 +1572,7 @@ int sphe_kernel_bytes(sphe_ctx* c, const char* kernel, double* bytes)
         const double n = (double)s.ids.size();
         N += n;
         int dims[3];
-        grid_dims(s.prm, dims);
+        grid_dims(s.prm, dims, c->yz);
         cells += (double)dims[0] * dims[1] * dims[2] + 1.0;
         tcells += (double)s.tnx * s.tnz;
         if (s.tnx > 0) terr_n += n;
@@ -1571,8 +1580,10 @@ int sphe_kernel_bytes(sphe_ctx* c, const char* kernel, double* bytes)
     }
     const std::string k = kernel;
     // compulsory inputs and outputs of each pass (intermediates kept in
-    // registers or on chip are not counted)
-    if (k == "density") *bytes = N * (3 * E + E) + 4 * cells;           // pos -> rho; start table
+    // registers or on chip are not counted; the cell table is the grid's
+    // own data, counted once in the scan, so no pass's figure depends on
+    // how finely the grid is cut)
+    if (k == "density") *bytes = N * (3 * E + E);                       // pos -> rho
     else if (k == "force")
         *bytes = N * (3 * E + 3 * E + E) + N * (3 * E + 3 * E) + N * 8  // pos, vel, rho -> pos, vel; key+rank
                  + terr_n * 4 + ero_n * (8 + 8);                        // heights; sediment in/out
@@ -1654,6 +1665,30 @@ int sphe_get_subcells(const sphe_ctx* c, int* slabs)
 {
     if (!c || !slabs) return SPHE_EARG;
     *slabs = 1 << (c->layout_valid ? c->D.sub_log2 : c->sub_log2);
+    return SPHE_OK;
+}
+
+int sphe_set_yz_split(sphe_ctx* c, int split)
+{
+    if (!c) return SPHE_EARG;
+    if (split != 1 && split != 2) return fail(c, SPHE_EARG, "yz split must be 1 or 2");
+    if (split == c->yz) return SPHE_OK;
+    for (const Scene& s : c->scenes) {
+        int dims[3];
+        grid_dims(s.prm, dims, split);
+        if ((double)dims[0] * dims[1] * dims[2] > 4.0e8) return fail(c, SPHE_ERANGE, "grid too fine for the box");
+    }
+    // a new cell grid: the state goes through the host and the layout is rebuilt
+    int r = begin_host_edit(c);
+    if (r) return r;
+    c->yz = split;
+    return SPHE_OK;
+}
+
+int sphe_get_yz_split(const sphe_ctx* c, int* split)
+{
+    if (!c || !split) return SPHE_EARG;
+    *split = c->yz;
     return SPHE_OK;
 }
 
@@ -1793,7 +1828,7 @@ int sphe_debug_consts(const sphe_ctx* c, int scene, double* out, int cap)
     if (!s || !out || cap < 20) return SPHE_EARG;
     SceneDev d;
     std::memset(&d, 0, sizeof(d));
-    derive(*s, d);
+    derive(*s, d, c->yz);
     const double v[20] = {d.f.h, d.f.h2, d.f.thr, d.f.t3h2, d.f.mc6, d.f.pf_scale, d.f.vf_scale,
                           d.f.ng, d.f.h * 1.00390625f, d.f.inv_cell, d.f.org[0], d.f.org[1],
                           d.f.org[2], (double)d.dims[0], (double)d.dims[1], (double)d.dims[2],

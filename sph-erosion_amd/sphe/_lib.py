@@ -102,6 +102,8 @@ SIGNATURES = {
     "sphe_get_list_capacity": (I, [P, C.POINTER(I)]),
     "sphe_set_subcells": (I, [P, I]),
     "sphe_get_subcells": (I, [P, C.POINTER(I)]),
+    "sphe_set_yz_split": (I, [P, I]),
+    "sphe_get_yz_split": (I, [P, C.POINTER(I)]),
     "sphe_debug_cell_keys": (I, [P, I, P]),
     "sphe_debug_consts": (I, [P, I, P, I]),
     "sphe_debug_neighbours": (I, [P, I, P, P, P]),

@@ -280,6 +280,15 @@ class Context:
         self._chk(lib().sphe_get_subcells(self._p, C.byref(v)), "get_subcells")
         return v.value
 
+    def set_yz_split(self, split):
+        """Cells 1/split of the x edge along y and z (1 or 2)."""
+        self._chk(lib().sphe_set_yz_split(self._p, split), "set_yz_split")
+
+    def get_yz_split(self):
+        v = C.c_int()
+        self._chk(lib().sphe_get_yz_split(self._p, C.byref(v)), "get_yz_split")
+        return v.value
+
     def autotune(self, steps_per_trial=5):
         self._chk(lib().sphe_autotune(self._p, steps_per_trial), "autotune")
         return self.get_tuning()

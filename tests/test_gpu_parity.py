@@ -290,6 +290,57 @@ def test_slab_split_bit_exact_twin(slabs):
     assert np.array_equal(ctx.sediment_fx(sid), st["sed"])
 
 
+@pytest.mark.parametrize("yz", [1])
+def test_yz_split_bit_exact_twin(yz):
+    """The y/z cell split sets the grid, the canonical order and the clipped
+    rows; the twin with the same split reproduces the GPU steps bit for bit
+    (the default split is covered by every other twin test)."""
+    sc = small_dam()
+    ctx, sid = gpu_context(sc)
+    ctx.set_yz_split(yz)
+    assert ctx.get_yz_split() == yz
+    pos, vel, sed, vol, height, fx = oracle_inputs(ctx, sid, sc)
+    osc = orc_scene_from(sc.params, sc.terrain, fx)
+    osc.yz = yz
+    counts, off, idx = ctx.debug_neighbours(sid)
+    bc, boff, bidx = O.neighbours_brute(sc.pos.astype(np.float32), O.derive(osc).thr)
+    assert np.array_equal(counts, bc) and np.array_equal(idx, bidx)
+    ctx.step(2)
+    gp, gv = ctx.state_f64(sid)
+    st = dict(pos=pos, vel=vel, sed=sed, vol=vol, height=height)
+    for _ in range(2):
+        st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], sc.ids, st["vol"], st["height"], real=4, comp=1)
+    assert np.array_equal(gp, st["pos"]) and np.array_equal(gv, st["vel"])
+    assert np.array_equal(ctx.sediment_fx(sid), st["sed"])
+
+
+def test_yz_split_change_keeps_the_state():
+    """Changing the split after steps rebuilds the layout from the pulled
+    state: the following steps equal the twin's with the new split from that
+    state, bit for bit (terrain, sediment and ids included)."""
+    sc = small_dam()
+    ctx, sid = gpu_context(sc)
+    ctx.set_yz_split(1)
+    _, _, _, _, _, fx = oracle_inputs(ctx, sid, sc)
+    ctx.step(3)
+    p3, v3 = ctx.state_f64(sid)
+    s3 = ctx.sediment_fx(sid)
+    vol3 = ctx.terrain_volume_fx(sid)
+    h3 = ctx.terrain_heights(sid)
+    ctx.set_yz_split(2)
+    assert ctx.get_yz_split() == 2
+    ctx.step(2)
+    gp, gv = ctx.state_f64(sid)
+    osc = orc_scene_from(sc.params, sc.terrain, fx)
+    osc.yz = 2
+    st = dict(pos=p3, vel=v3, sed=s3, vol=vol3.ravel(), height=h3.ravel())
+    for _ in range(2):
+        st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], sc.ids, st["vol"], st["height"], real=4, comp=1)
+    assert np.array_equal(gp, st["pos"]) and np.array_equal(gv, st["vel"])
+    assert np.array_equal(ctx.sediment_fx(sid), st["sed"])
+    assert np.array_equal(ctx.terrain_volume_fx(sid).ravel(), st["vol"].ravel())
+
+
 def test_crowded_cell_taken_whole():
     """A cell of 2300 particles (more than the 1023 the slab offsets can
     count) with probes along x through it: neighbour sets equal brute force
@@ -299,8 +350,8 @@ def test_crowded_cell_taken_whole():
     cell = float(h) * (1 + 2.0 ** -8)
     org = -0.2
     fr = np.concatenate([rng.uniform(0.02, 0.22, 1300), rng.uniform(0.78, 0.98, 1000)])
-    blob = org + cell * np.stack([2 + fr, 2 + rng.uniform(0.3, 0.7, 2300), 2 + rng.uniform(0.3, 0.7, 2300)], 1)
-    probes = org + cell * np.stack([rng.uniform(0.3, 4.7, 400), np.full(400, 2.5), np.full(400, 2.5)], 1)
+    blob = org + cell * np.stack([2 + fr, 2 + rng.uniform(0.3, 0.45, 2300), 2 + rng.uniform(0.3, 0.45, 2300)], 1)
+    probes = org + cell * np.stack([rng.uniform(0.3, 4.7, 400), np.full(400, 2.375), np.full(400, 2.375)], 1)
     pos = np.concatenate([blob, probes, rng.uniform(-0.19, 0.19, size=(1500, 3))]).astype(np.float32)
     n = len(pos)
     p = sphe.default_params(dt=1e-5)

@@ -21,11 +21,12 @@ def f3_scene():
     return O.make_scene(dt=1e-3, rho0=1000.0, g=(0.0, -9.8, 0.0), box_e=0.5)
 
 
-def test_grid_neighbours_equal_brute_force():
+@pytest.mark.parametrize("yz", [1, 2])
+def test_grid_neighbours_equal_brute_force(yz):
     rng = np.random.default_rng(11)
     pos = rng.uniform(-0.19, 0.19, size=(3000, 3)).astype(np.float32)
     pos[:300] = pos[300:600]  # coincident pairs
-    sc = O.make_scene(dt=0.0)
+    sc = O.make_scene(dt=0.0, yz=yz)
     n = len(pos)
     out = O.orc_step(sc, pos, np.zeros((n, 3)), np.zeros(n, np.int64), np.arange(n, dtype=np.int32),
                      real=4, comp=1)
@@ -44,11 +45,12 @@ def _clip_cases():
     h = np.float32(0.0457)
     cell = float(h) * (1 + 2.0 ** -8)
     org = -0.2
-    # cell (2, 2, 2) holds 2300 particles in its first and last x slab, and a
+    # cell (2, 2, 2) holds 2300 particles in its first and last x slab, all in
+    # one y/z cell also when cells are split along y and z, and a
     # line of probes along x through it ends its rows at every slab
     fr = np.concatenate([rng.uniform(0.02, 0.22, 1300), rng.uniform(0.78, 0.98, 1000)])
-    blob = org + cell * np.stack([2 + fr, 2 + rng.uniform(0.3, 0.7, 2300), 2 + rng.uniform(0.3, 0.7, 2300)], 1)
-    probes = org + cell * np.stack([rng.uniform(0.3, 4.7, 400), np.full(400, 2.5), np.full(400, 2.5)], 1)
+    blob = org + cell * np.stack([2 + fr, 2 + rng.uniform(0.3, 0.45, 2300), 2 + rng.uniform(0.3, 0.45, 2300)], 1)
+    probes = org + cell * np.stack([rng.uniform(0.3, 4.7, 400), np.full(400, 2.375), np.full(400, 2.375)], 1)
     blob = np.concatenate([blob, probes]).astype(np.float32)
     bg = rng.uniform(-0.19, 0.19, size=(1500, 3)).astype(np.float32)
     g = np.arange(-3, 4)
@@ -60,17 +62,18 @@ def _clip_cases():
             "edges": lat_c, "outside": np.concatenate([out, bg[:500]])}
 
 
+@pytest.mark.parametrize("yz", [1, 2])
 @pytest.mark.parametrize("slabs", [1, 2, 4])
 @pytest.mark.parametrize("case", ["blob", "lattice_h", "edges", "outside"])
 @pytest.mark.parametrize("real", [4, 8])
-def test_clipped_rows_keep_every_neighbour(slabs, case, real):
-    """The clipped stencil rows (x slabs, skipped rows, cells taken whole
-    above 1023 particles) still reach every neighbour the all-pairs FP32
-    predicate finds: per-particle counts and largest neighbour ids equal
-    brute force."""
+def test_clipped_rows_keep_every_neighbour(yz, slabs, case, real):
+    """The clipped stencil rows (x slabs, y/z cells of 1 or 1/2 the x edge,
+    skipped rows, cells taken whole above 1023 particles) still reach every
+    neighbour the all-pairs FP32 predicate finds: per-particle counts and
+    largest neighbour ids equal brute force."""
     pos = _clip_cases()[case]
     n = len(pos)
-    sc = O.make_scene(dt=0.0, slabs=slabs)
+    sc = O.make_scene(dt=0.0, slabs=slabs, yz=yz)
     out = O.orc_step(sc, pos, np.zeros((n, 3)), np.zeros(n, np.int64), np.arange(n, dtype=np.int32),
                      real=real, comp=1)
     thr = O.pred_threshold(np.float32(0.0457)) if real == 4 else float(np.float32(0.0457)) ** 2
@@ -88,19 +91,26 @@ def test_clipped_rows_keep_every_neighbour(slabs, case, real):
         assert np.array_equal(out["nbmax"], nbmax)
 
 
-def test_cell_keys_cover_neighbours():
-    """Every neighbour pair lies in adjacent cells (the 27-cell stencil)."""
+@pytest.mark.parametrize("yz", [1, 2])
+def test_cell_keys_cover_neighbours(yz):
+    """Every neighbour pair lies in adjacent cells along x and within yz
+    cells along y and z (the 3 x (2 yz + 1)^2 stencil), and the split cells
+    are exactly 1/yz of the x edge."""
     rng = np.random.default_rng(5)
     pos = rng.uniform(0.0, 1.0, size=(4000, 3)).astype(np.float32)
-    sc = O.make_scene(h=0.05, box_c=(0.5, 0.5, 0.5), box_e=0.5)
+    sc = O.make_scene(h=0.05, box_c=(0.5, 0.5, 0.5), box_e=0.5, yz=yz)
     c = O.derive(sc)
     keys = O.cell_keys(c, pos)
     nx, ny = c.dims[0], c.dims[1]
     k = np.stack([keys % nx, (keys // nx) % ny, keys // (nx * ny)], 1)
     cnt, off, idx = O.neighbours_brute(pos, c.thr)
+    reach = np.array([1, yz, yz])
     for i in range(0, 4000, 7):
         nb = idx[off[i]:off[i + 1]]
-        assert np.abs(k[nb] - k[i]).max() <= 1
+        assert (np.abs(k[nb] - k[i]) <= reach).all()
+    assert np.float32(c.inv_yz) == np.float32(c.inv_cell) * np.float32(yz)
+    c1 = O.derive(O.make_scene(h=0.05, box_c=(0.5, 0.5, 0.5), box_e=0.5, yz=1))
+    assert c.dims[0] == c1.dims[0] and c.dims[1] in (yz * c1.dims[1], yz * c1.dims[1] - 1)
 
 
 def test_fp64_grid_step_vs_reference_f3():
