@@ -236,49 +236,45 @@ __device__ __forceinline__ T gap_of(int d, T l1, T l2, T h1, T h2)
 }
 
 // Slot range [j0, j1) of the row at (dy, dz); false when the row is outside
-// the grid or out of the sphere's reach.
+// the grid or out of the sphere's reach.  The six table words a row may need
+// are loaded unconditionally (a skipped row reads the particle's own row),
+// so a row costs one memory round trip.
 template <typename T>
 __device__ __forceinline__ bool row_range(const DevState& D, const SceneDev& S, const RowProbe<T>& P, int dyc,
                                           int dzc, uint32_t& j0, uint32_t& j1)
 {
     const int nx = S.dims[0], ny = S.dims[1], nz = S.dims[2];
     const int z = P.kz + dzc, y = P.ky + dyc;
-    if (z < 0 || z >= nz || y < 0 || y >= ny) return false;
     const T mz = gap_of<T>(dzc, P.zl1, P.zl2, P.zh1, P.zh2);
     const T my = gap_of<T>(dyc, P.yl1, P.yl2, P.yh1, P.yh2);
     const T r2 = my * my + mz * mz;
-    if (r2 > T(1)) return false;
-    const T w = d_sqrt(T(1) - r2) + T(SPHE_ROW_MARGIN);
+    const bool on = z >= 0 && z < nz && y >= 0 && y < ny && r2 <= T(1);
+    const T w = d_sqrt(T(1) - (on ? r2 : T(0))) + T(SPHE_ROW_MARGIN);
     int flo = cidx<T>(d_floor((P.gx - w) * P.scale), P.fmax);
     int fhi = cidx<T>(d_floor((P.gx + w) * P.scale), P.fmax);
     flo = flo > P.flo_min ? flo : P.flo_min;
     fhi = fhi < P.fhi_max ? fhi : P.fhi_max;
-    const int64_t rk = (int64_t)S.cell_off + ((int64_t)z * ny + y) * nx;
+    const int64_t rk = (int64_t)S.cell_off + ((int64_t)(on ? z : P.kz) * ny + (on ? y : P.ky)) * nx;
     const int sl = P.sl, smax = P.smax;
     const int clo = flo >> sl, chi = fhi >> sl;
     const uint32_t* __restrict__ start = D.start;
     const uint32_t* __restrict__ subw = D.subw;
-    uint32_t a = start[rk + clo];
-    uint32_t b = start[rk + chi + 1];
-    if (sl) {
-        const int slo = flo & smax, shi = fhi & smax;
-        // the offsets are clamped to the cell: an empty cell's word is not
-        // rewritten each step, and any value then gives its empty range
-        if (slo) {
-            const uint32_t w0 = subw[rk + clo];
-            if (!(w0 >> 31)) a = min(a + ((w0 >> (10 * slo - 10)) & 1023u), start[rk + clo + 1]);
-        }
-        if (shi != smax) {
-            const uint32_t w1 = subw[rk + chi];
-            if (!(w1 >> 31)) b = min(start[rk + chi] + ((w1 >> (10 * shi)) & 1023u), start[rk + chi + 1]);
-        }
-    }
+    const uint32_t s_lo = start[rk + clo], s_lo1 = start[rk + clo + 1];
+    const uint32_t s_hi = start[rk + chi], s_hi1 = start[rk + chi + 1];
+    const uint32_t w0 = subw[rk + clo], w1 = subw[rk + chi];
+    // slab offsets (none with one slab: slo = 0, shi = smax), clamped to the
+    // cell: an empty cell's word is not rewritten each step, and any value
+    // then gives its empty range
+    const int slo = flo & smax, shi = fhi & smax;
+    uint32_t a = s_lo, b = s_hi1;
+    if (slo && !(w0 >> 31)) a = min(s_lo + ((w0 >> (10 * slo - 10)) & 1023u), s_lo1);
+    if (shi != smax && !(w1 >> 31)) b = min(s_hi + ((w1 >> (10 * shi)) & 1023u), s_hi1);
     // rows of real cells lie inside [p_begin, p_begin + n); the clamp only
     // matters if the counts were inconsistent
     const uint32_t lo = (uint32_t)S.p_begin, hi = (uint32_t)(S.p_begin + S.n);
     j0 = max(a, lo);
     j1 = min(b, hi);
-    return true;
+    return on;
 }
 
 // Calls row(j0, j1) for each row that may hold a neighbour, in row order.
