@@ -12,9 +12,11 @@ the library runs them by default, one HIP graph per step, inputs resident in
 HBM); the same K steps replayed from a saved state kernel by kernel, with
 each kernel's start and end in HIP events on the library's stream, for the
 per-kernel times and the HBM roofline of the dominant kernel (the replay
-must end in a bit-identical state); the density error against the FP64
-oracle; the plain FP64 GPU path over the same steps; and the reference step
-timed on the host's cores.
+must end in a bit-identical state); the density error against the
+library's FP64 path on the final state; the accuracy protocol (one step in
+FP32 compensated, FP32 plain and FP64 from a shared state); the plain FP64
+GPU path over the same steps; and the reference step timed on the host's
+cores (the CPU oracle's restatement of it, the only use of the oracle here).
 """
 import argparse
 import json
@@ -57,22 +59,90 @@ def make_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def density_error(sc, ctx, sid):
-    """Max relative density error of the GPU FP32 pass against the FP64 oracle."""
-    import oracle as O
-    p = sc.params
-    fx = ctx.fixed_exponent(sid)
-    osc = O.make_scene(h=p.h, mass=p.mass, rho0=p.rest_density, k=p.stiffness, visc=p.visc,
-                       surf_tens=p.surf_tens, g=tuple(p.gravity), dt=p.dt, box_c=tuple(p.box_center),
-                       box_e=p.box_half, tnx=sc.terrain["nx"], tnz=sc.terrain["nz"],
-                       t_org=tuple(sc.terrain["origin"]), t_cell=float(np.float32(sc.terrain["cell"])),
-                       fx_exp=fx, erosion=1, kc=p.kc, ke=p.ke, kd=p.kd, band=p.band)
-    pos, vel = ctx.state_f64(sid)
-    ids = ctx.ids(sid)
-    rho = ctx.debug_density(sid)
-    ref = O.orc_step(osc, pos, vel, ctx.sediment_fx(sid), ids, ctx.terrain_volume_fx(sid),
-                     ctx.terrain_heights(sid), real=8)["rho"]
+def snapshot(ctx, sids, f32=False):
+    """Per scene: positions, velocities (rounded to FP32 if f32), ids,
+    sediment and terrain volumes, as the library reads them back."""
+    out = []
+    for s in sids:
+        p, v = ctx.state_f64(s)
+        if f32:
+            p = p.astype(np.float32).astype(np.float64)
+            v = v.astype(np.float32).astype(np.float64)
+        nx, _ = ctx.terrain_dims(s)
+        out.append(dict(pos=p, vel=v, ids=np.asarray(ctx.ids(s), dtype=np.int32), sed=ctx.sediment_fx(s),
+                        vol=ctx.terrain_volume_fx(s) if nx else None))
+    return out
+
+
+def load_state(ctx, sids, states):
+    for s, st in zip(sids, states):
+        ctx.set_particles(s, st["pos"], vel=st["vel"], ids=st["ids"], sed=st["sed"])
+        if st["vol"] is not None:
+            ctx.set_terrain_volume_fx(s, st["vol"])
+
+
+def context_like(scs, precision, blk, rm):
+    import sphe
+    c = sphe.Context(precision)
+    sids = [c.add_synthetic(s) for s in scs]
+    c.set_tuning(blk, rm)
+    return c, sids
+
+
+def density_error(ctx, sids, scs, blk, rm):
+    """Max relative density error of the FP32 pass against the library's FP64
+    path on the same state (the state the timed region ended in)."""
+    import sphe
+    c64, s64 = context_like(scs, sphe.FP64, blk, rm)
+    load_state(c64, s64, snapshot(ctx, sids))
+    rho = np.concatenate([ctx.debug_density(s) for s in sids])
+    ref = np.concatenate([c64.debug_density(s) for s in s64])
+    c64.close()
     return float((np.abs(rho - ref) / ref).max())
+
+
+def accuracy_protocol(scs, blk, rm, warm=50):
+    """50 FP64 warm-up steps; the state rounded to FP32; from it, one step in
+    FP32 compensated, FP32 plain and FP64: density and acceleration of that
+    step, terrain height and sediment changes, each against FP64."""
+    import sphe
+    c64, s64 = context_like(scs, sphe.FP64, blk, rm)
+    c64.step(warm)
+    c64.sync()
+    st = snapshot(c64, s64, f32=True)
+    c64.close()
+    res = {}
+    for name, prec in (("fp64", sphe.FP64), ("fp32_compensated", sphe.FP32), ("fp32_plain", sphe.FP32_NAIVE)):
+        c, ss = context_like(scs, prec, blk, rm)
+        load_state(c, ss, st)
+        rho = np.concatenate([c.debug_density(s) for s in ss])
+        acc = np.concatenate([c.debug_accel(s) for s in ss])
+        h0 = [c.terrain_heights(s).astype(np.float64) for s in ss]
+        d0 = [c.sediment_fx(s) for s in ss]
+        c.step(1)
+        dh = np.concatenate([(c.terrain_heights(s) - h).ravel() for s, h in zip(ss, h0)])
+        ds = np.concatenate([c.sediment_fx(s) - d for s, d in zip(ss, d0)])
+        res[name] = (rho, acc, dh, ds)
+        c.close()
+    rho64, acc64, dh64, ds64 = res["fp64"]
+    n64 = np.linalg.norm(acc64, axis=1)
+    out = dict(protocol=f"{warm} FP64 warm-up steps, state rounded to FP32, one step per precision from it; "
+                        "errors against the FP64 step",
+               particles=int(len(rho64)))
+    for name in ("fp32_compensated", "fp32_plain"):
+        rho, acc, dh, ds = res[name]
+        rr = np.abs(rho - rho64) / rho64
+        ok = n64 > 0
+        ar = np.linalg.norm(acc - acc64, axis=1)[ok] / n64[ok]
+        out[name] = dict(rho_rel_max=float(rr.max()), rho_rel_p99=float(np.percentile(rr, 99)),
+                         acc_rel_p50=float(np.median(ar)), acc_rel_p99=float(np.percentile(ar, 99)),
+                         acc_rel_max=float(ar.max()),
+                         height_change_abs_max=float(np.abs(dh - dh64).max()) if dh.size else 0.0,
+                         height_change_fp64_max=float(np.abs(dh64).max()) if dh.size else 0.0,
+                         sediment_change_differs=int((ds != ds64).sum()),
+                         sediment_change_rel_max=(float(np.abs(ds - ds64).max() / np.abs(ds64).max())
+                                                  if ds.size and np.abs(ds64).max() > 0 else 0.0))
+    return out
 
 
 def cpu_baseline(sc, target_s=12.0):
@@ -116,7 +186,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--trace", type=int, default=0, help="steps to trace before the warmup (diagnostics)")
     ap.add_argument("--no-extras", action="store_true",
-                    help="only the timed region (for profiler runs): no statistics, oracle, FP64 or CPU legs")
+                    help="only the timed region (for profiler runs): no statistics, accuracy, FP64 or CPU legs")
     ap.add_argument("--mode", default="step", choices=["step", "copy"],
                     help="copy: time a plain 1 GiB device copy (HBM reference and counter calibration)")
     a = ap.parse_args()
@@ -226,9 +296,13 @@ def main():
                              p99=float(np.percentile(counts, 99)), max=int(counts.max()))
     out["density_over_rest"] = dict(mean=float((rho / rho0).mean()), max=float((rho / rho0).max()))
 
-    if len(scs) == 1 and N <= 300000:
-        log("bench: density error vs FP64 oracle")
-        out["density_rel_err_max"] = density_error(scs[0], ctx, sids[0])
+    log("bench: density error vs the FP64 path")
+    out["density_rel_err_max"] = density_error(ctx, sids, scs, blk, rm)
+    log("bench: accuracy protocol")
+    # after 50 steps as specified; after 400 the cfg2 column is on the
+    # terrain, so the height and sediment changes are compared too
+    out["accuracy"] = accuracy_protocol(scs, blk, rm, 50)
+    out["accuracy_400"] = accuracy_protocol(scs, blk, rm, 400)
 
     if not a.no_fp64:
         log("bench: FP64 plain GPU path")

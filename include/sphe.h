@@ -181,6 +181,12 @@ int sphe_terrain_load_u8(sphe_ctx* ctx, int scene, const uint8_t* img, int w, in
 int sphe_terrain_heights(sphe_ctx* ctx, int scene, float* out, size_t count);
 /* Fixed-point cell volumes, row = z. */
 int sphe_terrain_volume_fx(sphe_ctx* ctx, int scene, int64_t* out, size_t count);
+/* Restores fixed-point cell volumes read by sphe_terrain_volume_fx (count =
+ * nx * nz, 0 <= v <= 2^56, the scene's fixed exponent unchanged): with
+ * sphe_set_particles_f64 (sediment included) a checkpoint of a scene, from
+ * which the steps continue bit for bit.  No counterpart in the reference,
+ * whose terrain is static. */
+int sphe_set_terrain_volume_fx(sphe_ctx* ctx, int scene, const int64_t* vol, size_t count);
 int sphe_terrain_dims(const sphe_ctx* ctx, int scene, int* nx, int* nz);
 /* Exponent e of the scene's fixed-point volume unit 2^-e. */
 int sphe_fixed_exponent(const sphe_ctx* ctx, int scene, int* e);
@@ -291,6 +297,12 @@ int sphe_debug_neighbours(sphe_ctx* ctx, int scene, int32_t* counts, const int64
                           int32_t* idx);
 /* Density of the current state (one density pass, no step), id order. */
 int sphe_debug_density(sphe_ctx* ctx, int scene, double* rho);
+/* Acceleration (F / rho, fluid_system.h:318-321) of every particle in the
+ * current state, as the next step computes it (the same sums, bit for bit),
+ * without stepping: acc[3 i + a] in id order.  The reference exposes it per
+ * particle only, as FluidParticle::Acceleration through GetParticle
+ * (fluid_system.h:286-289). */
+int sphe_debug_accel(sphe_ctx* ctx, int scene, double* acc);
 /* Plain device-to-device streaming copy of `bytes` (a multiple of 16), run
  * `iters` times on the context's stream: HBM bandwidth reference and the
  * known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters.

@@ -1298,6 +1298,37 @@ __global__ void k_inspect(DevState D, int64_t slot, double* out)
     out[21] = (double)o.count;
 }
 
+// Acceleration of every particle of one scene in the current sorted state
+// (density pass done): the force pass's sums (list, pool or walk, the same
+// order and lane split) and finish(), without integration.
+// out[3 (slot - p_begin) + a], sorted order.
+template <typename T, bool COMP, int G>
+__global__ __launch_bounds__(256) void k_accel(DevState D, int scene, double* __restrict__ out)
+{
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t slot = gt / G;
+    const int l = (int)(gt & (G - 1));
+    if (slot >= D.P) return;
+    const int si = wave_scene(D.chunk_scene, slot);
+    if (si != scene) return;
+    const SceneDev& S = D.scenes[si];
+    if (slot >= S.p_begin + S.n) return;  // whole groups leave together
+    const SceneConsts<T>& K = SceneSel<T>::get(S);
+    const V4<T> pi = ((const V4<T>*)D.s_pos)[slot];
+    const V4<T> vi = ((const V4<T>*)D.s_vel)[slot];
+    const T rho_i = ((const T*)D.rho)[slot];
+    const T a_i = ((const V4<T>*)D.posa)[slot].w;
+    ForceAcc<T, COMP> acc;
+    if (l == 0) acc.self_term(K, vi.w);
+    force_share<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc);
+    acc.template merge<G>();
+    if (l != 0) return;
+    ForceOut<T> o;
+    acc.finish(K, rho_i, o);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[3 * (slot - S.p_begin) + a] = (double)o.acc[a];
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_neighbours(DevState D, int scene, const int64_t* __restrict__ off,
                                                     int32_t* __restrict__ counts, int32_t* __restrict__ idx)
@@ -1596,6 +1627,26 @@ void sphe_launch_inspect(const DevState& D, int64_t slot, int lanes, double* out
     case 2: launch_inspect_g<2>(D, slot, out, st); break;
     case 8: launch_inspect_g<8>(D, slot, out, st); break;
     default: launch_inspect_g<4>(D, slot, out, st); break;
+    }
+}
+
+template <int G>
+static void launch_accel_g(const DevState& D, int scene, double* out, hipStream_t st)
+{
+    const unsigned nb = grid_of(D.P * G, 256);
+    SPHE_PREC_SWITCH(D.precision, (k_accel<float, true, G><<<nb, 256, 0, st>>>(D, scene, out)),
+                     (k_accel<float, false, G><<<nb, 256, 0, st>>>(D, scene, out)),
+                     (k_accel<double, false, G><<<nb, 256, 0, st>>>(D, scene, out)));
+}
+
+void sphe_launch_accel(const DevState& D, int scene, int lanes, double* out, hipStream_t st)
+{
+    if (D.P == 0) return;
+    switch (lanes) {
+    case 1: launch_accel_g<1>(D, scene, out, st); break;
+    case 2: launch_accel_g<2>(D, scene, out, st); break;
+    case 8: launch_accel_g<8>(D, scene, out, st); break;
+    default: launch_accel_g<4>(D, scene, out, st); break;
     }
 }
 

@@ -23,6 +23,7 @@ void sphe_launch_clear_status(const DevState& D, hipStream_t st);
 void sphe_launch_terrain_mesh(const float* H, int nx, int nz, float cell, float ox, float oz, float* out,
                               hipStream_t st);
 void sphe_launch_inspect(const DevState& D, int64_t slot, int lanes, double* out, hipStream_t st);
+void sphe_launch_accel(const DevState& D, int scene, int lanes, double* out, hipStream_t st);
 void sphe_launch_neighbours(const DevState& D, int scene, const int64_t* off, int32_t* counts, int32_t* idx,
                             hipStream_t st);
 void sphe_launch_keys(const DevState& D, int scene, int32_t* keys, hipStream_t st);

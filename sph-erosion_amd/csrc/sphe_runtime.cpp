@@ -1375,6 +1375,19 @@ int sphe_terrain_volume_fx(sphe_ctx* c, int scene, int64_t* out, size_t count)
     return SPHE_OK;
 }
 
+int sphe_set_terrain_volume_fx(sphe_ctx* c, int scene, const int64_t* vol, size_t count)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    const size_t nt = (size_t)s->tnx * s->tnz;
+    if (!vol || nt == 0 || count != nt) return fail(c, SPHE_EARG, "expected %zu cell volumes", nt);
+    for (size_t t = 0; t < nt; ++t)
+        if (vol[t] < 0 || vol[t] > ((int64_t)1 << 56)) return fail(c, SPHE_ERANGE, "cell %zu volume out of range", t);
+    int r = begin_host_edit(c);
+    if (r) return r;
+    std::memcpy(s->vol.data(), vol, nt * 8);
+    return SPHE_OK;
+}
+
 int sphe_terrain_heights(sphe_ctx* c, int scene, float* out, size_t count)
 {
     SCENE_OR_FAIL(s, c, scene);
@@ -1902,6 +1915,33 @@ int sphe_debug_density(sphe_ctx* c, int scene, double* rho)
     c->stepped = false;
     if (r) return r;
     std::memcpy(rho, v.data(), v.size() * 8);
+    return SPHE_OK;
+}
+
+int sphe_debug_accel(sphe_ctx* c, int scene, double* acc)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (!acc) return fail(c, SPHE_EARG, "null output");
+    int r = sort_current(c);
+    if (r) return r;
+    const size_t n = s->ids.size();
+    if (n == 0) return SPHE_OK;
+    double* d = nullptr;
+    HIPCHK(c, hipMalloc(&d, n * 3 * 8));
+    sphe_launch_accel(c->D, scene, c->lanes, d, c->stream);
+    hipError_t e = hipGetLastError();
+    std::vector<double> v(n * 3);
+    std::vector<int32_t> id(n);
+    if (e == hipSuccess) e = hipMemcpyAsync(v.data(), d, n * 3 * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(id.data(), c->D.s_id + s->p_begin, n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(c, SPHE_EHIP, "debug_accel: %s", hipGetErrorString(e));
+    std::vector<size_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) { return id[a] < id[b]; });
+    for (size_t t = 0; t < n; ++t)
+        for (int a = 0; a < 3; ++a) acc[3 * t + a] = v[3 * perm[t] + a];
     return SPHE_OK;
 }
 

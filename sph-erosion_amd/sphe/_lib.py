@@ -108,6 +108,8 @@ SIGNATURES = {
     "sphe_debug_consts": (I, [P, I, P, I]),
     "sphe_debug_neighbours": (I, [P, I, P, P, P]),
     "sphe_debug_density": (I, [P, I, P]),
+    "sphe_debug_accel": (I, [P, I, P]),
+    "sphe_set_terrain_volume_fx": (I, [P, I, P, C.c_size_t]),
     "sphe_debug_copy": (I, [P, C.c_size_t, I, C.POINTER(C.c_double)]),
 }
 

@@ -193,6 +193,11 @@ class Context:
         self._chk(lib().sphe_terrain_volume_fx(self._p, scene, _ptr(out), out.size), "terrain_volume_fx")
         return out
 
+    def set_terrain_volume_fx(self, scene, vol):
+        """Restore fixed-point cell volumes (a checkpoint's terrain)."""
+        v = np.ascontiguousarray(vol, dtype=np.int64).ravel()
+        self._chk(lib().sphe_set_terrain_volume_fx(self._p, scene, _ptr(v), v.size), "set_terrain_volume_fx")
+
     def fixed_exponent(self, scene):
         e = C.c_int()
         self._chk(lib().sphe_fixed_exponent(self._p, scene, C.byref(e)), "fixed_exponent")
@@ -327,4 +332,10 @@ class Context:
     def debug_density(self, scene):
         out = np.zeros(self.num_particles(scene))
         self._chk(lib().sphe_debug_density(self._p, scene, _ptr(out)), "debug_density")
+        return out
+
+    def debug_accel(self, scene):
+        """Acceleration of every particle in the current state (id order, n x 3)."""
+        out = np.zeros((self.num_particles(scene), 3))
+        self._chk(lib().sphe_debug_accel(self._p, scene, _ptr(out)), "debug_accel")
         return out
