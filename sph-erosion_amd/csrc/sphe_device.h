@@ -15,6 +15,8 @@
 #include <stdint.h>
 
 #define SPHE_WAVE 64
+// cell counts per scan tile (256 threads x 16 counts)
+#define SPHE_SCAN_TILE_LOG2 12
 
 template <typename T>
 struct SceneConsts {
@@ -70,6 +72,10 @@ struct DevState {
     int32_t* key; uint32_t* arank;      // cell key << sub_log2 | x slab, arrival rank in the cell
     uint64_t* tmp_kid; int32_t* tmp_src;  // scatter output: (key, id) packed, previous slot
     uint32_t* cnt; uint32_t* start; uint32_t* partial; int64_t npartial;
+    // one word per scan tile of cnt (SPHE_SCAN_TILE cells): set when a count
+    // in the tile is taken, cleared by the scan that consumes the tile; the
+    // scan reads the counts of flagged tiles only (the rest are zero)
+    uint32_t* tocc;
     // cells are split into 2^sub_log2 slabs along x (sub_log2 <= 2); subw
     // holds each cell's slab offsets (10 bits per inner boundary; bit 31:
     // more than 1023 particles, no offsets), rewritten every step for the

@@ -332,6 +332,7 @@ __device__ __forceinline__ void count_position(const DevState& D, const SceneDev
     const int sub = sub_cell<T>(gx, kx, S.dims[0], D.sub_log2);
     D.key[slot] = (int32_t)((key << D.sub_log2) | sub);
     D.arank[slot] = atomicAdd(&D.cnt[key], 1u);
+    D.tocc[key >> SPHE_SCAN_TILE_LOG2] = 1u;
 }
 
 __device__ __forceinline__ void count_pad(const DevState& D, const SceneDev& S, int64_t slot)
@@ -339,6 +340,7 @@ __device__ __forceinline__ void count_pad(const DevState& D, const SceneDev& S, 
     const int64_t key = S.cell_off + S.ncell;
     D.key[slot] = (int32_t)(key << D.sub_log2);
     D.arank[slot] = atomicAdd(&D.cnt[key], 1u);
+    D.tocc[key >> SPHE_SCAN_TILE_LOG2] = 1u;
 }
 
 // ------------------------------------------------------------------ sort
@@ -360,6 +362,7 @@ __global__ __launch_bounds__(256) void k_hash_count(DevState D)
 
 #define SCAN_ITEMS 16
 #define SCAN_TILE (256 * SCAN_ITEMS)
+static_assert(SCAN_TILE == (1 << SPHE_SCAN_TILE_LOG2), "scan tile");
 
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uint32_t* lds)
 {
@@ -416,10 +419,16 @@ __device__ __forceinline__ void store_run(uint32_t* __restrict__ p, int64_t base
     }
 }
 
+// occ (optional): the tile flags; an unflagged tile's counts are all zero.
 __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
-                                                     uint32_t* __restrict__ partial)
+                                                     uint32_t* __restrict__ partial,
+                                                     const uint32_t* __restrict__ occ)
 {
     __shared__ uint32_t lds[4];
+    if (occ && occ[blockIdx.x] == 0u) {
+        if (threadIdx.x == 0) partial[blockIdx.x] = 0u;
+        return;
+    }
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     load_run(in, base, n, v);
@@ -434,16 +443,24 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
 // start[i] = exclusive prefix of counts.  With zero_counts the counts are
 // reset to zero for the next arrivals (the particle sort instead has each
 // particle's scatter reset its own cell); the optional single word zero is
-// reset too.
+// reset too.  occ (optional): the tile flags; an unflagged tile's counts are
+// not read (all zero), and each block clears its tile's flag once read.
 __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, uint32_t* __restrict__ start,
                                                    int64_t n, const uint32_t* __restrict__ partial,
-                                                   int zero_counts, unsigned long long* __restrict__ zero)
+                                                   int zero_counts, unsigned long long* __restrict__ zero,
+                                                   uint32_t* __restrict__ occ)
 {
     if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0ull;
     __shared__ uint32_t lds[8];
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    const bool empty = occ && occ[blockIdx.x] == 0u;  // block-uniform
     uint32_t v[SCAN_ITEMS];
-    load_run(cnt, base, n, v);
+    if (empty) {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i) v[i] = 0u;
+    } else {
+        load_run(cnt, base, n, v);
+    }
     uint32_t s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) s += v[i];
@@ -464,7 +481,9 @@ __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, u
         z[i] = 0u;
     }
     store_run(start, base, n, v);
-    if (zero_counts) store_run(cnt, base, n, z);
+    if (zero_counts && !empty) store_run(cnt, base, n, z);
+    // every thread read the flag before the barriers of the scans above
+    if (occ && !empty && threadIdx.x == 0) occ[blockIdx.x] = 0u;
 }
 
 __global__ __launch_bounds__(256) void k_scatter(DevState D)
@@ -1206,14 +1225,25 @@ __global__ __launch_bounds__(512) void k_force_f64(DevState D, int nblk, int rem
 
 // --------------------------------------------------------------- terrain
 
+// Terrain heights from the fixed-point volumes, four consecutive cells per
+// thread (16-byte loads and store).  A scene's range is padded to 64 cells,
+// so the four cells share a scene; cells past the scene's nx * nz are
+// padding and keep their value.
 __global__ __launch_bounds__(256) void k_heights(DevState D)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= D.TC) return;
-    const SceneDev& S = D.scenes[wave_scene(D.tchunk_scene, t)];
-    if (t - S.t_off >= (int64_t)S.tnx * S.tnz) return;
-    const float v = (float)D.vol[t];
-    D.height[t] = S.t_y0 + v * S.hq;
+    const int64_t t0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (t0 >= D.TC) return;
+    const SceneDev& S = D.scenes[D.tchunk_scene[t0 >> 6]];
+    const int64_t end = S.t_off + (int64_t)S.tnx * S.tnz;
+    const float y0 = S.t_y0, hq = S.hq;
+    if (t0 + 4 <= end) {
+        const longlong2 a = *(const longlong2*)(D.vol + t0);
+        const longlong2 b = *(const longlong2*)(D.vol + t0 + 2);
+        *(float4*)(D.height + t0) = make_float4(y0 + (float)a.x * hq, y0 + (float)a.y * hq,
+                                                y0 + (float)b.x * hq, y0 + (float)b.y * hq);
+    } else {
+        for (int64_t t = t0; t < end; ++t) D.height[t] = y0 + (float)D.vol[t] * hq;
+    }
 }
 
 // Two-phase erosion, phase 2a: count exchanges per terrain cell.
@@ -1514,12 +1544,13 @@ void sphe_launch_hash(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent
 }
 
 void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, bool zero_counts,
-                      unsigned long long* zero, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+                      unsigned long long* zero, uint32_t* occ, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    klaunch(k_scan_reduce, (unsigned)nb, 256, st, e0, nullptr, (const uint32_t*)cnt, n, partial);
+    klaunch(k_scan_reduce, (unsigned)nb, 256, st, e0, nullptr, (const uint32_t*)cnt, n, partial,
+            (const uint32_t*)occ);
     klaunch(k_scan_down, (unsigned)nb, 256, st, nullptr, e1, cnt, start, n, (const uint32_t*)partial,
-            zero_counts ? 1 : 0, zero);
+            zero_counts ? 1 : 0, zero, occ);
 }
 
 int64_t sphe_scan_partials(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
@@ -1589,14 +1620,14 @@ void sphe_launch_force(const DevState& D, int block, int remap, int lanes, hipSt
 void sphe_launch_heights(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     if (D.TC == 0) return;
-    klaunch(k_heights, grid_of(D.TC, 256), 256, st, e0, e1, D);
+    klaunch(k_heights, grid_of(D.TC / 4, 256), 256, st, e0, e1, D);  // TC: a multiple of 64
 }
 
 void sphe_launch_two_phase(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     if (D.TC == 0 || D.P == 0) return;
     klaunch(k_tcount, grid_of(D.P, 256), 256, st, e0, nullptr, D);
-    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, true, nullptr, st);
+    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, true, nullptr, nullptr, st);
     k_tscatter<<<grid_of(D.P, 256), 256, 0, st>>>(D);
     klaunch(k_tapply, grid_of(D.TC, 256), 256, st, nullptr, e1, D);
 }

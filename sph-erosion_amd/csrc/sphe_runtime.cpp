@@ -409,6 +409,7 @@ int build_layout(sphe_ctx* c)
     while (D.sub_log2 > 0 && (C << D.sub_log2) > INT32_MAX) --D.sub_log2;  // keys are int32
     D.npartial = sphe_scan_partials(C + 1);
     DALLOC(D.partial, (size_t)D.npartial * 4);
+    DALLOC(D.tocc, (size_t)D.npartial * 4);
     DALLOC(D.vol, (size_t)TC * 8);
     DALLOC(D.height, (size_t)TC * 4);
     DALLOC(D.dq, Pn * 8);
@@ -449,6 +450,7 @@ int build_layout(sphe_ctx* c)
     HIPCHK(c, hipMemcpyAsync(cs, hcs.data(), hcs.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(tcs, htcs.data(), htcs.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(D.cnt, 0, (size_t)(C + 1) * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(D.tocc, 0, (size_t)D.npartial * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(D.subw, 0, (size_t)(C + 1) * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(D.tcnt, 0, (size_t)(TC + 1) * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(D.status, 0, 64, c->stream));
@@ -686,7 +688,7 @@ void launch_sort(sphe_ctx* c)
     const DevState& D = c->D;
     {
         Timed t(c, "scan");
-        sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, false, D.ptop, c->stream, t.a, t.b);
+        sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, false, D.ptop, D.tocc, c->stream, t.a, t.b);
     }
     {
         Timed t(c, "scatter");

@@ -233,15 +233,19 @@ def cfg5(count=1024, erosion=EROSION_ATOMIC, big_n=8388608, steps=2000, big_T=40
 def cfg5_share(gpus=8, erosion=EROSION_ATOMIC, big_n=8388608, steps=2000, big_T=4096, base_idx=5000):
     """One GPU's share of cfg5's 8,388,608-particle scene split into `gpus` x-slabs: the
     particles of the first slab (x < 1/gpus) plus a ghost layer one cell (h (1 + 2^-8)) wide
-    beyond it, on the same 4096^2 terrain (generated from the same seed as cfg5's)."""
+    beyond it, over the matching columns of the same 4096^2 terrain (generated from the same
+    seed as cfg5's; the slab's columns plus two)."""
     rng2 = np.random.default_rng(scene_seed(base_idx))
     s = (0.1 / big_n) ** (1.0 / 3.0)
     h = 2.15 * s
     ter = make_terrain(big_T, 6, 0.3, 0.1, rng2)
     y0 = float(ter["heights"].max()) + s
     pos, meta = lattice_block(big_n, s, 0.0, 1.0, 0.0, 1.0, y0, rng2)
-    keep = pos[:, 0] < 1.0 / gpus + h * (1.0 + 2.0 ** -8)
+    edge = 1.0 / gpus + h * (1.0 + 2.0 ** -8)
+    keep = pos[:, 0] < edge
     pos = pos[keep]
+    cols = min(big_T, int(np.ceil(edge * big_T)) + 2)
+    ter = dict(ter, nx=cols, heights=np.ascontiguousarray(ter["heights"][:, :cols]))
     n = len(pos)
     return Scene(f"cfg5_share_{n}", sph_params(h, s, 5e-4, erosion), ter, pos, np.zeros_like(pos),
                  np.arange(n, dtype=np.int32), steps, dict(meta, s=s, h=h, slab=1.0 / gpus))

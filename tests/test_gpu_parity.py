@@ -500,3 +500,68 @@ def test_checkpoint_continues_bit_for_bit():
         b.set_terrain_volume_fx(sb, vol.ravel()[:-1])
     with pytest.raises(Exception):
         b.set_terrain_volume_fx(sb, -np.ones(vol.size, np.int64))
+
+
+def test_full_size_column_properties():
+    """The 262,144-particle column (the bench workload) after 40 steps:
+    exact mass units; a replay from a saved state ends bit for bit where the
+    first run did; density within 1e-5 of the FP64 path on the same state;
+    symmetric neighbour sets (fl(a - b) = -fl(b - a), so j is i's neighbour
+    exactly when i is j's)."""
+    sc = scenes.cfg2()
+    ctx, sid = gpu_context(sc)
+    _, _, u0 = ctx.mass(sid)
+    ctx.step(40)
+    ctx.save_state()
+    ctx.step(10)
+    p1, v1 = ctx.state_f64(sid)
+    ctx.restore_state()
+    ctx.step(10)
+    p2, v2 = ctx.state_f64(sid)
+    assert np.array_equal(p1, p2) and np.array_equal(v1, v2)
+    _, _, u1 = ctx.mass(sid)
+    assert u1 == u0
+    c64, s64 = gpu_context(sc, sphe.FP64)
+    c64.set_particles(s64, p2, vel=v2, ids=ctx.ids(sid), sed=ctx.sediment_fx(sid))
+    c64.set_terrain_volume_fx(s64, ctx.terrain_volume_fx(sid))
+    rho = ctx.debug_density(sid)
+    ref = c64.debug_density(s64)
+    assert (np.abs(rho - ref) / ref).max() <= DENSITY_TOL
+    counts, off, idx = ctx.debug_neighbours(sid)
+    n = len(counts)
+    assert counts.min() >= 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), counts)
+    fwd = rows * n + idx.astype(np.int64)
+    rev = idx.astype(np.int64) * n + rows
+    assert np.array_equal(np.sort(fwd), np.sort(rev))
+
+
+def test_empty_scene_and_lone_particle():
+    """A scene without particles beside a populated one changes nothing; a
+    lone particle (itself its only neighbour) steps bit for bit like the
+    twin and accelerates at g to within one ulp."""
+    sc = small_dam()
+    a, sa = gpu_context(sc)
+    b = sphe.Context()
+    se = b.add_scene(sphe.default_params(dt=1e-3))
+    sb = b.add_synthetic(sc)
+    a.step(3)
+    b.step(3)
+    pa, va = a.state_f64(sa)
+    pb, vb = b.state_f64(sb)
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+    assert b.num_particles(se) == 0
+    prm = sphe.default_params(dt=1e-3)
+    c = sphe.Context()
+    s1 = c.add_scene(prm)
+    pos = np.array([[0.0, 0.05, 0.0]])
+    c.set_particles(s1, pos, vel=np.zeros((1, 3)), ids=np.array([7]))
+    acc = c.debug_accel(s1)[0]
+    g = np.float32(prm.gravity[1])
+    assert acc[0] == 0.0 and acc[2] == 0.0
+    assert abs(np.float32(acc[1]) - g) <= np.spacing(abs(g))
+    c.step(1)
+    p, v = c.state_f64(s1)
+    twin = O.orc_step(orc_scene_from(prm), pos, np.zeros((1, 3)), np.zeros(1, np.int64),
+                      np.array([7], np.int32), None, None, real=4, comp=1)
+    assert np.array_equal(p, twin["pos"]) and np.array_equal(v, twin["vel"])
