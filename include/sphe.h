@@ -157,6 +157,13 @@ int sphe_set_particles_f64(sphe_ctx* ctx, int scene, int n, const double* pos3, 
 int sphe_num_particles(const sphe_ctx* ctx, int scene, int* n);
 /* Copies one per-particle buffer (SPHE_BUF_*) in ascending id order. */
 int sphe_read_buffer(sphe_ctx* ctx, int scene, int which, void* out, size_t bytes);
+/* The same buffer as a host view owned by the context: *view points at
+ * *count records (3 floats for POS and VEL, else 1 float or int32), a
+ * snapshot that stays valid until the next sphe_buffer call for the same
+ * scene and buffer or sphe_destroy.  The reference hands out views of its
+ * per-particle fields (FluidParticle members through GetParticle,
+ * fluid_system.h:286-289; the FPOS .. FPRESS field ids, :22-26). */
+int sphe_buffer(sphe_ctx* ctx, int scene, int which, const void** view, size_t* count);
 /* Positions and velocities in FP64 (FP64 context) in ascending id order. */
 int sphe_read_state_f64(sphe_ctx* ctx, int scene, double* pos3, double* vel3);
 /* Fixed-point sediment (1 unit = 2^-sphe_fixed_exponent volume), id order. */
@@ -208,6 +215,14 @@ int sphe_terrain_mesh_device(sphe_ctx* ctx, int scene, const float** dverts, siz
 /* FluidSystemSPH::Run (fluid_system.h:104-183) for every scene, nsteps
  * times.  Returns after the work is queued; sphe_sync waits for it. */
 int sphe_step(sphe_ctx* ctx, int nsteps);
+/* A batch: one context holding n scenes (params[i], terrains[i] or none if
+ * terrains is NULL), which every step advances together, one launch per
+ * kernel for all of them.  On an error nothing is created.  No counterpart
+ * in the reference (one FluidSystemSPH per scene). */
+int sphe_batch_create(int precision, const sphe_params* params, const sphe_terrain_desc* terrains, int n,
+                      sphe_ctx** out);
+/* Advances every scene of a batch nsteps times: sphe_step. */
+int sphe_batch_step(sphe_ctx* ctx, int nsteps);
 int sphe_sync(sphe_ctx* ctx);
 /* Capture the step sequence in a HIP graph (default on). */
 int sphe_use_graph(sphe_ctx* ctx, int enable);

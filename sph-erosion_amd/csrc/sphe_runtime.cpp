@@ -85,6 +85,7 @@ struct sphe_ctx {
     std::vector<ProfRec> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<double, int>> prof;
+    std::map<std::pair<int, int>, std::vector<float>> views;  // sphe_buffer snapshots by (scene, buffer)
 
     int block = 128;
     int remap = 128;  // density: runs of 128 consecutive blocks per XCD (L2 locality); force: plain order
@@ -1251,6 +1252,21 @@ int sphe_read_buffer(sphe_ctx* c, int scene, int which, void* out, size_t bytes)
     return SPHE_OK;
 }
 
+int sphe_buffer(sphe_ctx* c, int scene, int which, const void** view, size_t* count)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (!view) return fail(c, SPHE_EARG, "null view");
+    const size_t n = s->ids.size();
+    const size_t comps = (which == SPHE_BUF_POS || which == SPHE_BUF_VEL) ? 3 : 1;
+    std::vector<float>& v = c->views[std::make_pair(scene, which)];
+    v.assign(std::max<size_t>(1, n * comps), 0.0f);  // float and int32 records are 4 bytes
+    int r = sphe_read_buffer(c, scene, which, v.data(), v.size() * 4);
+    if (r) return r;
+    *view = v.data();
+    if (count) *count = n;
+    return SPHE_OK;
+}
+
 int sphe_read_state_f64(sphe_ctx* c, int scene, double* pos3, double* vel3)
 {
     SCENE_OR_FAIL(s, c, scene);
@@ -1488,6 +1504,28 @@ int sphe_terrain_mesh_device(sphe_ctx* c, int scene, const float** dverts, size_
     *dverts = d;
     return SPHE_OK;
 }
+
+int sphe_batch_create(int precision, const sphe_params* params, const sphe_terrain_desc* terrains, int n,
+                      sphe_ctx** out)
+{
+    if (!out || n < 0 || (n > 0 && !params)) return SPHE_EARG;
+    *out = nullptr;
+    sphe_ctx* c = nullptr;
+    int r = sphe_create(precision, &c);
+    if (r) return r;
+    for (int i = 0; i < n; ++i) {
+        int sid = 0;
+        r = sphe_add_scene(c, &params[i], terrains ? &terrains[i] : nullptr, &sid);
+        if (r) {
+            (void)sphe_destroy(c);
+            return r;
+        }
+    }
+    *out = c;
+    return SPHE_OK;
+}
+
+int sphe_batch_step(sphe_ctx* c, int nsteps) { return sphe_step(c, nsteps); }
 
 int sphe_step(sphe_ctx* c, int nsteps)
 {

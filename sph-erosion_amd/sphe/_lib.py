@@ -71,6 +71,9 @@ SIGNATURES = {
     "sphe_set_particles_f64": (I, [P, I, I, P, P, P, P]),
     "sphe_num_particles": (I, [P, I, C.POINTER(I)]),
     "sphe_read_buffer": (I, [P, I, I, P, C.c_size_t]),
+    "sphe_buffer": (I, [P, I, I, C.POINTER(P), C.POINTER(C.c_size_t)]),
+    "sphe_batch_create": (I, [I, P, P, I, C.POINTER(P)]),
+    "sphe_batch_step": (I, [P, I]),
     "sphe_read_state_f64": (I, [P, I, P, P]),
     "sphe_read_sediment_fx": (I, [P, I, P]),
     "sphe_get_particle": (I, [P, I, I, C.POINTER(Particle)]),

@@ -137,6 +137,17 @@ class Context:
         self._chk(lib().sphe_read_buffer(self._p, scene, which, _ptr(out), out.nbytes), "read_buffer")
         return out
 
+    def view(self, scene, which):
+        """sphe_buffer: the context's host view of a buffer (copied out here)."""
+        v = C.c_void_p()
+        n = C.c_size_t()
+        self._chk(lib().sphe_buffer(self._p, scene, which, C.byref(v), C.byref(n)), "buffer")
+        comps = 3 if which in (BUF_POS, BUF_VEL) else 1
+        ctype = C.c_int32 if which == BUF_ID else C.c_float
+        arr = np.ctypeslib.as_array(C.cast(v, C.POINTER(ctype)), shape=(max(1, n.value * comps),))
+        out = arr[:n.value * comps].copy()
+        return out.reshape(n.value, 3) if comps == 3 else out
+
     def positions(self, scene):
         return self.read(scene, BUF_POS)
 

@@ -88,3 +88,51 @@ def test_terrain_mesh_after_erosion_matches_oracle():
     assert sphe.lib().sphe_terrain_mesh_device(ctx._p, sid, C.byref(dptr), C.byref(n)) == 0
     assert n.value == ref.size and dptr.value
     assert np.array_equal(_hip_copy(dptr.value, n.value * 4).view(np.uint32), ref.view(np.uint32))
+
+
+def test_buffer_views_and_batches():
+    """sphe_buffer's host views equal sphe_read_buffer's copies; a batch made
+    by sphe_batch_create (terrains included) and stepped by sphe_batch_step
+    equals a context given the same scenes one by one, bit for bit."""
+    import ctypes as C
+
+    from sphe import _lib as L
+
+    scs = [scenes.cfg1(seed_idx=k, n=1500 + 700 * k) for k in range(3)]
+    ctx = sphe.Context()
+    sids = [ctx.add_synthetic(s) for s in scs]
+    ctx.step(3)
+    for s in sids:
+        for which in (sphe.BUF_POS, sphe.BUF_VEL, sphe.BUF_DENSITY, sphe.BUF_PRESS, sphe.BUF_SED,
+                      sphe.BUF_ID, sphe.BUF_MASS):
+            assert np.array_equal(ctx.view(s, which), ctx.read(s, which))
+    params = (sphe.Params * 3)(*[s.params for s in scs])
+    tds = (sphe.TerrainDesc * 3)()
+    hts = []
+    for k, s in enumerate(scs):
+        h = np.ascontiguousarray(s.terrain["heights"], dtype=np.float32)
+        hts.append(h)
+        tds[k].nx, tds[k].nz = int(s.terrain["nx"]), int(s.terrain["nz"])
+        for a in range(3):
+            tds[k].origin[a] = float(s.terrain["origin"][a])
+        tds[k].cell = float(s.terrain["cell"])
+        tds[k].heights = h.ctypes.data
+    b = C.c_void_p()
+    assert L.lib().sphe_batch_create(sphe.FP32, C.byref(params), C.byref(tds), 3, C.byref(b)) == 0
+    bc = sphe.Context.__new__(sphe.Context)
+    bc.precision, bc._p, bc._terrains = sphe.FP32, b, hts
+    try:
+        for k, s in enumerate(scs):
+            bc.set_particles(k, s.pos, s.vel, s.ids)
+        ref = sphe.Context()
+        for s in scs:
+            ref.add_synthetic(s)
+        assert L.lib().sphe_batch_step(b, 4) == 0
+        ref.step(4)
+        for k in range(3):
+            pa, va = bc.state_f64(k)
+            pb, vb = ref.state_f64(k)
+            assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+            assert np.array_equal(bc.terrain_volume_fx(k), ref.terrain_volume_fx(k))
+    finally:
+        bc.close()
