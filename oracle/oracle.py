@@ -97,6 +97,7 @@ def lib():
         L.orc_pred_threshold.argtypes = [C.c_float]
         L.orc_pred_threshold.restype = C.c_float
         L.orc_root_f32_n.argtypes = [P, C.c_int, P, P]
+        L.orc_row_width_f32_n.argtypes = [P, C.c_int, P]
         L.orc_cell_keys.argtypes = [C.POINTER(OrcConsts), P, C.c_int, P]
         L.orc_neighbours_brute.argtypes = [P, C.c_int, C.c_float, P, P, P]
         L.orc_neighbours_brute.restype = C.c_int64
@@ -214,6 +215,14 @@ def derive(scene):
 
 def pred_threshold(h):
     return lib().orc_pred_threshold(float(np.float32(h)))
+
+
+def row_width_f32(x):
+    """The FP32 kernels' clipped-row half-width (before the margin)."""
+    x = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    w = np.empty_like(x)
+    lib().orc_row_width_f32_n(x.ctypes.data, len(x), w.ctypes.data)
+    return w
 
 
 def root_f32(d2):

@@ -14,6 +14,9 @@
  *                      plus closed forms (isolated particle, lattice density).
  *   orc_pred_threshold fluid_system.h:118-119 + glm length.  Pinned by an
  *                      exhaustive ulp walk against sqrtf and brute force.
+ *   orc_row_width_f32  build-defined row half-width (Newton 1/sqrt, one
+ *                      step).  Pinned by a dense sweep of [0, 1] (within
+ *                      2e-3 of sqrt) and the clipped-row superset tests.
  *   orc_root_f32       length and normalize of fluid_system.h:118, 438-441
  *                      as the FP32 kernels evaluate them (Newton 1/sqrt).
  *                      Pinned by ulp bounds against sqrtf and 1/sqrtf over
@@ -315,6 +318,28 @@ void orc_root_f32(float d2, float* r, float* inv)
     *r = d2 * y;
 }
 
+/* Half-width of a clipped stencil row before its margin, as the FP32 kernels
+ * compute it (build-defined): x / sqrt(x) with one Newton step for 1/sqrt(x)
+ * from the integer estimate, within 2e-3 of sqrt(x) on [0, 1] (the 2^-6 row
+ * margin covers it).  Pinned by a dense sweep of [0, 1] and by the
+ * clipped-row superset tests. */
+float orc_row_width_f32(float x)
+{
+    int32_t bits;
+    memcpy(&bits, &x, sizeof bits);
+    bits = 0x5f375a86 - (bits >> 1);
+    float y;
+    memcpy(&y, &bits, sizeof y);
+    const float t = (0.5f * x) * y;
+    y = y * fmaf(-t, y, 1.5f);
+    return x * y;
+}
+
+void orc_row_width_f32_n(const float* x, int n, float* w)
+{
+    for (int i = 0; i < n; i++) w[i] = orc_row_width_f32(x[i]);
+}
+
 void orc_root_f32_n(const float* d2, int n, float* r, float* inv)
 {
     for (int i = 0; i < n; i++) orc_root_f32(d2[i], &r[i], &inv[i]);
@@ -541,6 +566,7 @@ static int cmp_sortrec(const void* a, const void* b)
 #define OT double
 #define OSFX _d
 #define OT_ROOT(d2, r, inv) do { (r) = sqrt(d2); (inv) = 1.0 / (r); } while (0)
+#define OT_ROW_WIDTH(x) sqrt(x)
 #define OT_SQRT sqrt
 #define OT_FLOOR floor
 #define OT_FABS fabs
@@ -549,6 +575,7 @@ static int cmp_sortrec(const void* a, const void* b)
 #undef OT
 #undef OSFX
 #undef OT_ROOT
+#undef OT_ROW_WIDTH
 #undef OT_SQRT
 #undef OT_FLOOR
 #undef OT_FABS
@@ -557,6 +584,7 @@ static int cmp_sortrec(const void* a, const void* b)
 #define OT float
 #define OSFX _f
 #define OT_ROOT(d2, r, inv) orc_root_f32((d2), &(r), &(inv))
+#define OT_ROW_WIDTH(x) orc_row_width_f32(x)
 #define OT_SQRT sqrtf
 #define OT_FLOOR floorf
 #define OT_FABS fabsf
@@ -565,6 +593,7 @@ static int cmp_sortrec(const void* a, const void* b)
 #undef OT
 #undef OSFX
 #undef OT_ROOT
+#undef OT_ROW_WIDTH
 #undef OT_SQRT
 #undef OT_FLOOR
 #undef OT_FABS

@@ -136,6 +136,10 @@ float orc_pred_threshold(float h);
 void orc_root_f32(float d2, float* r, float* inv);
 void orc_root_f32_n(const float* d2, int n, float* r, float* inv);
 
+/* Half-width of a clipped stencil row before its margin (FP32 kernels). */
+float orc_row_width_f32(float x);
+void orc_row_width_f32_n(const float* x, int n, float* w);
+
 /* Cell key (x fastest) of each position, FP32. */
 void orc_cell_keys(const orc_consts* c, const float* pos3, int n, int32_t* keys);
 

@@ -76,12 +76,17 @@ __device__ __forceinline__ V at(const V* __restrict__ base, uint32_t j)
     return *(const V*)((const char*)base + j * (uint32_t)sizeof(V));
 }
 
+__device__ __forceinline__ float d_fmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double d_fmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ float d_fmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double d_fmin(double a, double b) { return __builtin_fmin(a, b); }
+
+// A floored cell coordinate clamped to [0, nmax]; negative and NaN give 0
+// (max(NaN, 0) = 0), branch-free.
 template <typename T>
 __device__ __forceinline__ int cidx(T f, int nmax)
 {
-    if (!(f >= T(0))) return 0;  // negative and NaN
-    if (f >= T(nmax)) return nmax;
-    return (int)f;
+    return (int)d_fmin(d_fmax(f, T(0)), T(nmax));
 }
 
 template <typename T>
@@ -175,6 +180,25 @@ __device__ __forceinline__ T row_gap(T f)
 // position and include a margin far above the rounding of the cell
 // coordinates.  Cells with more than 1023 particles carry no slab offsets
 // and are taken whole.
+// Half-width of a clipped row, in x-cell units, before the margin: at least
+// sqrt(x) - 2e-3 for x in [0, 1].  FP64: sqrt.  FP32: x / sqrt(x) with one
+// Newton step y <- y (1.5 - (x/2) y^2) for 1/sqrt(x) from the integer
+// estimate 0x5f375a86 - bits(x)/2 (relative error at most 1.75e-3); the row
+// margin (2^-6) covers the difference, and the sequence is IEEE operations
+// only, so a CPU replays the same rows.
+template <typename T> struct RowWidth {
+    static __device__ __forceinline__ T of(T x) { return d_sqrt(x); }
+};
+template <> struct RowWidth<float> {
+    static __device__ __forceinline__ float of(float x)
+    {
+        float y = __int_as_float(0x5f375a86 - (__float_as_int(x) >> 1));
+        const float t = (0.5f * x) * y;
+        y = y * __builtin_fmaf(-t, y, 1.5f);
+        return x * y;
+    }
+};
+
 template <typename T>
 struct RowProbe {
     T gx, scale;
@@ -182,6 +206,7 @@ struct RowProbe {
     // above (h1, h2) along y and z
     T yl1, yl2, yh1, yh2, zl1, zl2, zh1, zh2;
     int kx, ky, kz, sl, smax, fmax, flo_min, fhi_max;
+    uint32_t rbase;  // global index of the first cell of the particle's own row
 };
 
 // r / W for 0 <= r < W^2 and W = 3 or 5 (multiply and shift)
@@ -210,6 +235,7 @@ __device__ __forceinline__ RowProbe<T> row_probe(const DevState& D, const SceneD
     P.kx = cidx<T>(d_floor(P.gx), nx - 1);
     P.ky = cidx<T>(d_floor(gy), ny - 1);
     P.kz = cidx<T>(d_floor(gz), nz - 1);
+    P.rbase = (uint32_t)S.cell_off + (uint32_t)((P.kz * ny + P.ky) * nx);
     const T fy = gy - T(P.ky), fz = gz - T(P.kz);
     const T q = K.yz_scale;  // 1 / yz, exact
     P.yl1 = row_gap<T>(fy * q);
@@ -232,43 +258,52 @@ __device__ __forceinline__ RowProbe<T> row_probe(const DevState& D, const SceneD
 template <typename T>
 __device__ __forceinline__ T gap_of(int d, T l1, T l2, T h1, T h2)
 {
-    return d == 0 ? T(0) : (d == -1 ? l1 : (d == 1 ? h1 : (d < 0 ? l2 : h2)));
+    // selects of values at hand (no branches): -1 l1, 1 h1, -2 l2, 2 h2, 0
+    const T one = d < 0 ? l1 : h1;
+    const T two = d < 0 ? l2 : h2;
+    const T g = (d == 2 || d == -2) ? two : one;
+    return d == 0 ? T(0) : g;
 }
 
 // Slot range [j0, j1) of the row at (dy, dz); false when the row is outside
-// the grid or out of the sphere's reach.  The six table words a row may need
+// the grid or out of the sphere's reach.  roff = (dz ny + dy) nx, the row's
+// cell offset from the particle's own row (mod 2^32), kept by the callers.  The six table words a row may need
 // are loaded unconditionally (a skipped row reads the particle's own row),
 // so a row costs one memory round trip.
 template <typename T>
 __device__ __forceinline__ bool row_range(const DevState& D, const SceneDev& S, const RowProbe<T>& P, int dyc,
-                                          int dzc, uint32_t& j0, uint32_t& j1)
+                                          int dzc, uint32_t roff, uint32_t& j0, uint32_t& j1)
 {
-    const int nx = S.dims[0], ny = S.dims[1], nz = S.dims[2];
+    const int ny = S.dims[1], nz = S.dims[2];
     const int z = P.kz + dzc, y = P.ky + dyc;
     const T mz = gap_of<T>(dzc, P.zl1, P.zl2, P.zh1, P.zh2);
     const T my = gap_of<T>(dyc, P.yl1, P.yl2, P.yh1, P.yh2);
     const T r2 = my * my + mz * mz;
-    const bool on = z >= 0 && z < nz && y >= 0 && y < ny && r2 <= T(1);
-    const T w = d_sqrt(T(1) - (on ? r2 : T(0))) + T(SPHE_ROW_MARGIN);
+    const bool on = (z >= 0) & (z < nz) & (y >= 0) & (y < ny) & (r2 <= T(1));
+    const T w = RowWidth<T>::of(T(1) - (on ? r2 : T(0))) + T(SPHE_ROW_MARGIN);
     int flo = cidx<T>(d_floor((P.gx - w) * P.scale), P.fmax);
     int fhi = cidx<T>(d_floor((P.gx + w) * P.scale), P.fmax);
-    flo = flo > P.flo_min ? flo : P.flo_min;
-    fhi = fhi < P.fhi_max ? fhi : P.fhi_max;
-    const int64_t rk = (int64_t)S.cell_off + ((int64_t)(on ? z : P.kz) * ny + (on ? y : P.ky)) * nx;
+    flo = max(flo, P.flo_min);
+    fhi = min(fhi, P.fhi_max);
+    // cell indices are below 2^30 (the layout's limit): 32-bit offsets
+    const uint32_t rk = on ? P.rbase + roff : P.rbase;
     const int sl = P.sl, smax = P.smax;
-    const int clo = flo >> sl, chi = fhi >> sl;
-    const uint32_t* __restrict__ start = D.start;
-    const uint32_t* __restrict__ subw = D.subw;
-    const uint32_t s_lo = start[rk + clo], s_lo1 = start[rk + clo + 1];
-    const uint32_t s_hi = start[rk + chi], s_hi1 = start[rk + chi + 1];
-    const uint32_t w0 = subw[rk + clo], w1 = subw[rk + chi];
+    const uint32_t clo = rk + (uint32_t)(flo >> sl), chi = rk + (uint32_t)(fhi >> sl);
+    const uint32_t s_lo = at(D.start, clo), s_lo1 = at(D.start, clo + 1);
+    const uint32_t s_hi = at(D.start, chi), s_hi1 = at(D.start, chi + 1);
+    const uint32_t w0 = at(D.subw, clo), w1 = at(D.subw, chi);
     // slab offsets (none with one slab: slo = 0, shi = smax), clamped to the
     // cell: an empty cell's word is not rewritten each step, and any value
-    // then gives its empty range
+    // then gives its empty range; starts never decrease, so min(s_lo, s_lo1)
+    // is s_lo when there is no offset
     const int slo = flo & smax, shi = fhi & smax;
-    uint32_t a = s_lo, b = s_hi1;
-    if (slo && !(w0 >> 31)) a = min(s_lo + ((w0 >> (10 * slo - 10)) & 1023u), s_lo1);
-    if (shi != smax && !(w1 >> 31)) b = min(s_hi + ((w1 >> (10 * shi)) & 1023u), s_hi1);
+    // field shifts from byte tables: 10 (slo - 1) for slo = 1..3, 10 shi
+    const uint32_t sh0 = (0x140A0000u >> ((uint32_t)slo << 3)) & 0xFFu;
+    const uint32_t sh1 = (0x1E140A00u >> ((uint32_t)shi << 3)) & 0xFFu;
+    const uint32_t f0 = (slo != 0 && !(w0 >> 31)) ? (w0 >> sh0) & 1023u : 0u;
+    const bool use1 = shi != smax && !(w1 >> 31);
+    const uint32_t a = min(s_lo + f0, s_lo1);
+    const uint32_t b = use1 ? min(s_hi + ((w1 >> sh1) & 1023u), s_hi1) : s_hi1;
     // rows of real cells lie inside [p_begin, p_begin + n); the clamp only
     // matters if the counts were inconsistent
     const uint32_t lo = (uint32_t)S.p_begin, hi = (uint32_t)(S.p_begin + S.n);
@@ -284,11 +319,14 @@ __device__ __forceinline__ void for_each_row(const DevState& D, const SceneDev& 
 {
     const RowProbe<T> P = row_probe<T>(D, S, K, xi, yi, zi);
     const int yz = D.yz;
-    for (int dz = -yz; dz <= yz; ++dz)
-        for (int dy = -yz; dy <= yz; ++dy) {
+    const int nx = S.dims[0], nyx = S.dims[1] * nx;
+    for (int dz = -yz; dz <= yz; ++dz) {
+        uint32_t roff = (uint32_t)(dz * nyx - yz * nx);
+        for (int dy = -yz; dy <= yz; ++dy, roff += (uint32_t)nx) {
             uint32_t j0, j1;
-            if (row_range<T>(D, S, P, dy, dz, j0, j1)) row(j0, j1);
+            if (row_range<T>(D, S, P, dy, dz, roff, j0, j1)) row(j0, j1);
         }
+    }
 }
 
 // Visits the candidates of the clipped stencil rows in ascending slot order
@@ -631,14 +669,24 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     {
         const RowProbe<T> P = row_probe<T>(D, S, K, pi.x, pi.y, pi.z);
         uint32_t mine = 0;
+        // row r = W dz + dy (dz, dy from 0): lane l starts at row l and steps
+        // by G, carrying dz, dy and the row's cell offset (no multiplies)
+        const int nx = S.dims[0], nyx = S.dims[1] * nx;
+        int dz = row_div(l, W), dy = l - dz * W;
+        uint32_t roff = (uint32_t)((dz - yz) * nyx + (dy - yz) * nx);
         for (int r = l; r < nrows; r += G) {
-            const int dz = row_div(r, W);
-            const int dy = r - dz * W;
             uint32_t j0 = 0, j1 = 0;
-            const bool on = row_range<T>(D, S, P, dy - yz, dz - yz, j0, j1);
+            const bool on = row_range<T>(D, S, P, dy - yz, dz - yz, roff, j0, j1);
             const uint32_t len = on && j1 > j0 ? j1 - j0 : 0u;
             tab[r] = make_uint2(j0, len);
             mine += len;
+            dy += G;
+            roff += (uint32_t)(G * nx);
+            while (dy >= W) {
+                dy -= W;
+                ++dz;
+                roff += (uint32_t)(nyx - W * nx);
+            }
         }
         // the group's lanes are one wave's: order its LDS writes before the reads
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -217,6 +217,17 @@ def test_fp32_root_is_within_ulps_of_sqrt():
     assert (np.float32(h) - rs == h).all()
 
 
+def test_fp32_row_width_within_margin():
+    """The FP32 clipped-row half-width is within 2e-3 of sqrt(x) on [0, 1]
+    (the 2^-6 row margin covers it; a planted error above the margin fails
+    the clipped-row superset tests), exact at 0 and finite everywhere."""
+    x = np.concatenate([np.linspace(0.0, 1.0, 400_001), np.exp2(-np.arange(0, 149, 0.25))]).astype(np.float32)
+    w = O.row_width_f32(x)
+    assert np.isfinite(w).all()
+    assert np.abs(w.astype(np.float64) - np.sqrt(x.astype(np.float64))).max() <= 2e-3
+    assert O.row_width_f32(np.zeros(1))[0] == 0.0
+
+
 def test_neighbour_brute_force_matches_reference_predicate():
     rng = np.random.default_rng(3)
     pos = rng.uniform(-0.1, 0.1, size=(400, 3)).astype(np.float32)
