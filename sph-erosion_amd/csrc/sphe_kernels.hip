@@ -98,6 +98,29 @@ __device__ __forceinline__ void cell_of(const SceneConsts<T>& K, const int32_t* 
     kz = cidx<T>(d_floor((z - K.org[2]) * K.inv_yz), dims[2] - 1);
 }
 
+// Value of lane l ^ M.  Within a quad (M = 1, 2) a DPP quad permute (ALU
+// only); otherwise a wave shuffle.
+template <int M>
+__device__ __forceinline__ float lane_xor(float v)
+{
+    if constexpr (M == 1) return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+    else if constexpr (M == 2) return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+    else return __shfl_xor(v, M, SPHE_WAVE);
+}
+template <int M>
+__device__ __forceinline__ double lane_xor(double v)
+{
+    if constexpr (M <= 2) {
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+        constexpr int ctrl = M == 1 ? 0xB1 : 0x4E;  // quad_perm [1,0,3,2] / [2,3,0,1]
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)b, ctrl, 0xF, 0xF, false);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), ctrl, 0xF, 0xF, false);
+        return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+    } else {
+        return __shfl_xor(v, M, SPHE_WAVE);
+    }
+}
+
 // Neighbour-sum accumulators.  A particle's sum is split over the G lanes of
 // its group (lane l takes the candidates at positions l, l+G, ... of each
 // stencil row); merge_xor(m) combines the partials of lanes l and l^m, so a
@@ -115,11 +138,12 @@ template <typename T> struct Acc<T, true> {
     }
     // TwoSum of the heads (exact error e), compensations added: the result
     // is the same in both lanes of the pair.
-    __device__ __forceinline__ void merge_xor(int m)
+    template <int M>
+    __device__ __forceinline__ void merge_xor()
     {
-        const T os = __shfl_xor(s, m, SPHE_WAVE);
-        const T oc = __shfl_xor(c, m, SPHE_WAVE);
-        const bool low = (threadIdx.x & m) == 0;
+        const T os = lane_xor<M>(s);
+        const T oc = lane_xor<M>(c);
+        const bool low = (threadIdx.x & M) == 0;
         const T a = low ? s : os, b = low ? os : s;
         const T ca = low ? c : oc, cb = low ? oc : c;
         const T sum = a + b;
@@ -133,10 +157,11 @@ template <typename T> struct Acc<T, true> {
 template <typename T> struct Acc<T, false> {
     T s = T(0);
     __device__ __forceinline__ void add(T v) { s = s + v; }
-    __device__ __forceinline__ void merge_xor(int m)
+    template <int M>
+    __device__ __forceinline__ void merge_xor()
     {
-        const T os = __shfl_xor(s, m, SPHE_WAVE);
-        const bool low = (threadIdx.x & m) == 0;
+        const T os = lane_xor<M>(s);
+        const bool low = (threadIdx.x & M) == 0;
         s = low ? s + os : os + s;
     }
     __device__ __forceinline__ T value() const { return s; }
@@ -145,8 +170,9 @@ template <typename T> struct Acc<T, false> {
 template <int G, typename A>
 __device__ __forceinline__ void group_merge(A& a)
 {
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) a.merge_xor(m);
+    if constexpr (G > 1) a.template merge_xor<1>();
+    if constexpr (G > 2) a.template merge_xor<2>();
+    if constexpr (G > 4) a.template merge_xor<4>();
 }
 
 // x sub-cell of a position inside its cell: the cell is split into
@@ -1080,14 +1106,6 @@ __device__ __forceinline__ int64_t erode_exchange(const SceneDev& S, const Scene
     return dq;
 }
 
-// Lane l's share of the force sums from the sub-list the density pass wrote
-// (its stencil positions l, l+G, ... that lie inside h, in order, the
-// particle itself left out).  Only ~1 in 8 stencil candidates lies inside h,
-// so walking the stencil again would run the ~100-instruction pair body for
-// the whole wave whenever any lane accepts; the list holds accepted pairs
-// only.  Two neighbours per iteration keep four gathers in flight.  A lane
-// whose share overflowed the list walks its stencil positions instead (same
-// pairs, same order, same result).
 // Lane l's share of the force sums from list entries l, l+G, l+2G, ... of
 // a particle's neighbour sequence (fetch(k) = its k-th neighbour): the lanes
 // of a particle differ by one neighbour at most.  Two pairs per iteration
