@@ -726,10 +726,10 @@ void launch_step(sphe_ctx* c)
 std::string plan_signature(const sphe_ctx* c)
 {
     char b[256];
-    snprintf(b, sizeof(b), "%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%p/%d/%d", c->precision, c->block, c->remap,
+    snprintf(b, sizeof(b), "%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%p/%d/%d/%d", c->precision, c->block, c->remap,
              c->lanes, (long long)c->D.P, (long long)c->D.C, (long long)c->D.TC, (int)any_erosion(c, -1),
              (int)any_erosion(c, SPHE_EROSION_TWO_PHASE), (const void*)c->D.u_pos, (const void*)c->D.nbl,
-             (const void*)c->D.pool, c->D.maxn, c->D.sub_log2);
+             (const void*)c->D.pool, c->D.maxn, c->D.sub_log2, c->D.yz);
     return b;
 }
 
