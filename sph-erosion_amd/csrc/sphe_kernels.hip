@@ -2,14 +2,17 @@
 //
 // One step (all scenes of a context at once), reference Erosion/fluid_system.h
 // FluidSystemSPH::Run (:104-183) + advance (:306-353):
-//   scan      counts -> cell starts (2 kernels; also re-zeroes the counts)
-//   scatter   each particle to its cell segment (arrival order)
+//   scan      counts -> cell starts (2 kernels; tiles no particle counted
+//             into are not read)
+//   scatter   each particle to its cell segment (arrival order); resets
+//             its cell's count for the next step
 //   reorder   rank inside the cell by (x slab, particle id) -> canonical
 //             (cell, slab, id) order; gathers the state into sorted SoA
 //             buffers and writes the cell's slab offsets
 //   density   poly6 density + pressure terms over the stencil rows clipped
-//             to the smoothing sphere (fluid_system.h:109-124), and the
-//             neighbour lists for the force pass
+//             to the smoothing sphere (fluid_system.h:109-124; cells are
+//             h (1 + 2^-8) along x and half that along y and z, 25 rows),
+//             and the neighbour lists for the force pass
 //   force     pressure, viscosity, colour normal/Laplacian (:129-178),
 //             symplectic Euler + box response (:318-350, :355-407),
 //             heightfield collision and erosion exchange (build-defined),
@@ -122,9 +125,10 @@ __device__ __forceinline__ double lane_xor(double v)
 }
 
 // Neighbour-sum accumulators.  A particle's sum is split over the G lanes of
-// its group (lane l takes the candidates at positions l, l+G, ... of each
-// stencil row); merge_xor(m) combines the partials of lanes l and l^m, so a
-// xor tree over m = 1, 2, .. G/2 leaves the whole sum in every lane.
+// its group (density: lane l takes positions l, l+G, ... of the candidate
+// stream; forces: list entries l, l+G, ...); merge_xor<M>() combines the
+// partials of lanes l and l^M, so a xor tree over M = 1, 2, .. G/2 leaves
+// the whole sum in every lane.
 template <typename T, bool COMP> struct Acc;
 template <typename T> struct Acc<T, true> {
     // Kahan: the represented sum is s - c
@@ -185,8 +189,8 @@ __device__ __forceinline__ int sub_cell(T gx, int kx, int nx, int sl)
     return cidx<T>(d_floor(gx * T(1 << sl)), (nx << sl) - 1) - (kx << sl);
 }
 
-// Lower bound, in cell units, of the distance from a coordinate at fraction
-// f of its cell to the next cell, less a margin for rounding.
+// A row gap g (a lower bound of the distance to a row, in x-cell units)
+// less the rounding margin, floored at 0.
 #define SPHE_ROW_MARGIN 0.015625
 template <typename T>
 __device__ __forceinline__ T row_gap(T f)

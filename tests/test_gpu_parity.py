@@ -76,11 +76,15 @@ def test_density_vs_fp64_and_twin(precision, comp):
     assert np.array_equal(rho.astype(np.float32), twin.astype(np.float32))
 
 
-@pytest.mark.parametrize("erosion", [sphe.EROSION_OFF, sphe.EROSION_ATOMIC, sphe.EROSION_TWO_PHASE])
-def test_step_fp32_bit_exact_twin(erosion):
+@pytest.mark.parametrize("erosion,precision,comp", [(sphe.EROSION_OFF, sphe.FP32, 1),
+                                                     (sphe.EROSION_ATOMIC, sphe.FP32, 1),
+                                                     (sphe.EROSION_TWO_PHASE, sphe.FP32, 1),
+                                                     (sphe.EROSION_ATOMIC, sphe.FP32_NAIVE, 0)])
+def test_step_fp32_bit_exact_twin(erosion, precision, comp):
+    """Three steps equal the twin's bit for bit (positions, velocities,
+    sediment, terrain volumes and heights), compensated and plain FP32."""
     sc = small_dam(erosion=erosion)
-    # moving state: a first step from rest, then compare the second
-    ctx, sid = gpu_context(sc)
+    ctx, sid = gpu_context(sc, precision)
     pos, vel, sed, vol, height, fx = oracle_inputs(ctx, sid, sc)
     osc = orc_scene_from(sc.params, sc.terrain, fx)
     ctx.step(3)
@@ -90,7 +94,7 @@ def test_step_fp32_bit_exact_twin(erosion):
     gh = ctx.terrain_heights(sid)
     st = dict(pos=pos, vel=vel, sed=sed, vol=vol, height=height)
     for _ in range(3):
-        out = O.orc_step(osc, st["pos"], st["vel"], st["sed"], sc.ids, st["vol"], st["height"], real=4, comp=1)
+        out = O.orc_step(osc, st["pos"], st["vel"], st["sed"], sc.ids, st["vol"], st["height"], real=4, comp=comp)
         st = out
     assert np.array_equal(gp, st["pos"]), np.abs(gp - st["pos"]).max()
     assert np.array_equal(gv, st["vel"]), np.abs(gv - st["vel"]).max()
