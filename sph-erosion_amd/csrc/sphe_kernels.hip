@@ -487,17 +487,17 @@ __device__ __forceinline__ void store_run(uint32_t* __restrict__ p, int64_t base
     }
 }
 
-// occ (optional): the tile flags; an unflagged tile's counts are all zero.
-__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
-                                                     uint32_t* __restrict__ partial,
-                                                     const uint32_t* __restrict__ occ)
+// Total of count tile `tile` into partial[tile].  occ (optional): the tile
+// flags; an unflagged tile's counts are all zero.
+__device__ __forceinline__ void scan_reduce_tile(const uint32_t* __restrict__ in, int64_t n,
+                                                 uint32_t* __restrict__ partial, const uint32_t* __restrict__ occ,
+                                                 int tile, uint32_t* lds)
 {
-    __shared__ uint32_t lds[4];
-    if (occ && occ[blockIdx.x] == 0u) {
-        if (threadIdx.x == 0) partial[blockIdx.x] = 0u;
+    if (occ && occ[tile] == 0u) {
+        if (threadIdx.x == 0) partial[tile] = 0u;
         return;
     }
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    const int64_t base = (int64_t)tile * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     load_run(in, base, n, v);
     uint32_t s = 0;
@@ -505,7 +505,15 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
     for (int i = 0; i < SCAN_ITEMS; ++i) s += v[i];
     uint32_t tot;
     block_exclusive_scan(s, &tot, lds);
-    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+    if (threadIdx.x == 0) partial[tile] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
+                                                     uint32_t* __restrict__ partial,
+                                                     const uint32_t* __restrict__ occ)
+{
+    __shared__ uint32_t lds[4];
+    scan_reduce_tile(in, n, partial, occ, (int)blockIdx.x, lds);
 }
 
 // start[i] = exclusive prefix of counts.  With zero_counts the counts are
@@ -1299,10 +1307,8 @@ __global__ __launch_bounds__(512) void k_force_f64(DevState D, int nblk, int rem
 // thread (16-byte loads and store).  A scene's range is padded to 64 cells,
 // so the four cells share a scene; cells past the scene's nx * nz are
 // padding and keep their value.
-__global__ __launch_bounds__(256) void k_heights(DevState D)
+__device__ __forceinline__ void heights_quad(const DevState& D, int64_t t0)
 {
-    const int64_t t0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-    if (t0 >= D.TC) return;
     const SceneDev& S = D.scenes[D.tchunk_scene[t0 >> 6]];
     const int64_t end = S.t_off + (int64_t)S.tnx * S.tnz;
     const float y0 = S.t_y0, hq = S.hq;
@@ -1314,6 +1320,27 @@ __global__ __launch_bounds__(256) void k_heights(DevState D)
     } else {
         for (int64_t t = t0; t < end; ++t) D.height[t] = y0 + (float)D.vol[t] * hq;
     }
+}
+
+__global__ __launch_bounds__(256) void k_heights(DevState D)
+{
+    const int64_t t0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (t0 < D.TC) heights_quad(D, t0);
+}
+
+// The particle scan's first kernel and the terrain heights in one launch
+// (both need only the force pass done): blocks [0, nscan) total the count
+// tiles, the others compute heights from the volumes the last force pass
+// eroded, before this step's force pass samples them.
+__global__ __launch_bounds__(256) void k_scan_reduce_heights(DevState D, int nscan)
+{
+    __shared__ uint32_t lds[4];
+    if ((int)blockIdx.x >= nscan) {  // block-uniform: no barrier is skipped by part of a block
+        const int64_t t0 = ((int64_t)(blockIdx.x - nscan) * blockDim.x + threadIdx.x) * 4;
+        if (t0 < D.TC) heights_quad(D, t0);
+        return;
+    }
+    scan_reduce_tile(D.cnt, D.C + 1, D.partial, D.tocc, (int)blockIdx.x, lds);
 }
 
 // Two-phase erosion, phase 2a: count exchanges per terrain cell.
@@ -1685,6 +1712,16 @@ void sphe_launch_force(const DevState& D, int block, int remap, int lanes, hipSt
     case 8: launch_force_g<8>(D, block, remap, st, e0, e1); break;
     default: launch_force_g<4>(D, block, remap, st, e0, e1); break;
     }
+}
+
+void sphe_launch_scan_heights(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+{
+    const int64_t n = D.C + 1;
+    const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    const int64_t nh = (int64_t)grid_of(D.TC / 4, 256);  // TC: a multiple of 64
+    klaunch(k_scan_reduce_heights, (unsigned)(nb + nh), 256, st, e0, nullptr, D, (int)nb);
+    klaunch(k_scan_down, (unsigned)nb, 256, st, nullptr, e1, D.cnt, D.start, n, (const uint32_t*)D.partial, 0,
+            D.ptop, D.tocc);
 }
 
 void sphe_launch_heights(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)

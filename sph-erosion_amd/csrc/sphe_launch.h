@@ -13,6 +13,8 @@ void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* parti
                       unsigned long long* zero, uint32_t* occ, hipStream_t st, hipEvent_t e0 = nullptr,
                       hipEvent_t e1 = nullptr);
 int64_t sphe_scan_partials(int64_t n);
+// The particle scan with the terrain heights computed in its first launch.
+void sphe_launch_scan_heights(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void sphe_launch_scatter(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void sphe_launch_reorder(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void sphe_launch_density(const DevState& D, int block, int remap, int lanes, hipStream_t st,

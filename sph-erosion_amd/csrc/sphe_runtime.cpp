@@ -69,6 +69,7 @@ struct sphe_ctx {
     bool device_valid = false;  // device holds the latest state
     bool host_valid = true;     // host staging holds the latest state
     bool counted = false;       // key / arank / cnt prepared for the next scan
+    bool heights_stale = false; // the last step eroded; heights follow in the next step's scan
     bool stepped = false;       // sorted buffers hold the last step's pre-step state
 
     DevState D{};
@@ -340,6 +341,7 @@ void free_device(sphe_ctx* c)
     c->counted = false;
     c->stepped = false;
     c->table_uploaded = false;
+    c->heights_stale = false;
 }
 
 int dalloc(sphe_ctx* c, void** p, size_t bytes)
@@ -527,6 +529,7 @@ int upload(sphe_ctx* c)
     sphe_launch_heights(D, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->heights_stale = false;
     c->device_valid = true;
     c->counted = false;
     c->stepped = false;
@@ -584,6 +587,17 @@ int begin_host_edit(sphe_ctx* c)
     c->device_valid = false;
     c->stepped = false;
     c->layout_valid = false;
+    return SPHE_OK;
+}
+
+// Heights of the volumes the last step eroded (the next step's scan would
+// compute them), for readers between steps.
+int ensure_heights(sphe_ctx* c)
+{
+    if (!c->heights_stale || !c->device_valid) return SPHE_OK;
+    sphe_launch_heights(c->D, c->stream);
+    HIPCHK(c, hipGetLastError());
+    c->heights_stale = false;
     return SPHE_OK;
 }
 
@@ -689,7 +703,10 @@ void launch_sort(sphe_ctx* c)
     const DevState& D = c->D;
     {
         Timed t(c, "scan");
-        sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, false, D.ptop, D.tocc, c->stream, t.a, t.b);
+        // with erosion the terrain heights of the last force pass are computed
+        // in the scan's first launch (this step's force pass samples them)
+        if (any_erosion(c, -1) && D.TC > 0) sphe_launch_scan_heights(D, c->stream, t.a, t.b);
+        else sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, false, D.ptop, D.tocc, c->stream, t.a, t.b);
     }
     {
         Timed t(c, "scatter");
@@ -716,10 +733,6 @@ void launch_step(sphe_ctx* c)
     if (any_erosion(c, SPHE_EROSION_TWO_PHASE)) {
         Timed t(c, "two_phase");
         sphe_launch_two_phase(D, c->stream, t.a, t.b);
-    }
-    if (any_erosion(c, -1)) {
-        Timed t(c, "heights");
-        sphe_launch_heights(D, c->stream, t.a, t.b);
     }
 }
 
@@ -845,6 +858,7 @@ int do_steps(sphe_ctx* c, int nsteps)
         for (int k = k0; k < k1; ++k) {
             if (graph) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
             else launch_step(c);
+            if (any_erosion(c, -1)) c->heights_stale = true;
         }
         HIPCHK(c, hipGetLastError());
         c->stepped = true;
@@ -1413,6 +1427,7 @@ int sphe_terrain_heights(sphe_ctx* c, int scene, float* out, size_t count)
     if (!out || count < nt) return fail(c, SPHE_EARG, "output too small");
     if (nt == 0) return SPHE_OK;
     int r = ensure_device(c);
+    if (!r) r = ensure_heights(c);
     if (r) return r;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(out, c->D.height + s->t_off, nt * 4, hipMemcpyDeviceToHost));
@@ -1425,6 +1440,7 @@ int mesh_on_device(sphe_ctx* c, const Scene* s, float** out)
 {
     const size_t nv = (size_t)s->tnx * s->tnz * 6;
     int r = ensure_device(c);
+    if (!r) r = ensure_heights(c);
     if (r) return r;
     if (nv > c->mesh_cap) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
