@@ -649,6 +649,18 @@ __device__ __forceinline__ uint32_t group_prefix(uint32_t x, int l, uint32_t* to
     return y;
 }
 
+// Running sums of v's bytes 0..2 in bytes 0..2 of the result (b0, b0+b1,
+// b0+b1+b2; no carries while they stay below 256): the 24-bit product with
+// 0x010101, one full-rate v_mul_u32_u24.  Written as that instruction: the
+// compiler otherwise folds a following shift and add into a 64-bit
+// multiply-add, a quarter-rate instruction.
+__device__ __forceinline__ uint32_t byte_runs(uint32_t v)
+{
+    uint32_t r;
+    asm("v_mul_u32_u24_e32 %0, 0x10101, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
 // G = 4: the groups are the wave's quads; three DPP quad permutes.
 template <>
 __device__ __forceinline__ uint32_t group_prefix<4>(uint32_t x, int l, uint32_t* tot)
@@ -698,26 +710,36 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     bool claimed = false;  // the group asked for a chunk (at most once)
     uint32_t stream_total = 0;
     // The particle's clipped rows as one stream of candidates, t = 0 ..
-    // total-1 in canonical order.  Lane l computes rows l, l+G, ... and
-    // writes (first slot, length) to the group's row table in LDS (dynamic:
-    // blockDim / G tables of W^2 entries).
+    // total-1 in canonical order.  Lane l computes rows l, l+G, ... (the
+    // group takes G rows per round) and writes (stream end, slot offset) to
+    // the group's row table in LDS (dynamic: blockDim / G tables of W^2 + 1
+    // entries): positions t of row r lie below its end and hold slot t +
+    // offset.  A group prefix sum per round gives the stream ends.  Entry
+    // W^2 is a sentinel row that ends at 2^32 - 1, so a walk along the table
+    // stops by itself.
     const int yz = D.yz, W = 2 * yz + 1, nrows = W * W;
     extern __shared__ uint2 rowtab[];
-    uint2* __restrict__ tab = rowtab + (threadIdx.x / G) * nrows;
+    uint2* __restrict__ tab = rowtab + (threadIdx.x / G) * (nrows + 1);
     {
         const RowProbe<T> P = row_probe<T>(D, S, K, pi.x, pi.y, pi.z);
-        uint32_t mine = 0;
+        uint32_t ends = 0;  // stream end of the rounds so far (all lanes)
         // row r = W dz + dy (dz, dy from 0): lane l starts at row l and steps
         // by G, carrying dz, dy and the row's cell offset (no multiplies)
         const int nx = S.dims[0], nyx = S.dims[1] * nx;
         int dz = row_div(l, W), dy = l - dz * W;
         uint32_t roff = (uint32_t)((dz - yz) * nyx + (dy - yz) * nx);
-        for (int r = l; r < nrows; r += G) {
-            uint32_t j0 = 0, j1 = 0;
-            const bool on = row_range<T>(D, S, P, dy - yz, dz - yz, roff, j0, j1);
-            const uint32_t len = on && j1 > j0 ? j1 - j0 : 0u;
-            tab[r] = make_uint2(j0, len);
-            mine += len;
+        const int rounds = (nrows + G - 1) / G;
+        for (int k = 0, r = l; k < rounds; ++k, r += G) {
+            uint32_t j0 = 0, len = 0;
+            if (r < nrows) {
+                uint32_t j1 = 0;
+                const bool on = row_range<T>(D, S, P, dy - yz, dz - yz, roff, j0, j1);
+                len = on && j1 > j0 ? j1 - j0 : 0u;
+            }
+            uint32_t rtot;
+            const uint32_t end = ends + group_prefix<G>(len, l, &rtot);
+            if (r < nrows) tab[r] = make_uint2(end, j0 - (end - len));
+            ends += rtot;
             dy += G;
             roff += (uint32_t)(G * nx);
             while (dy >= W) {
@@ -726,42 +748,38 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
                 roff += (uint32_t)(nyx - W * nx);
             }
         }
+        if (l == 0) tab[nrows] = make_uint2(~0u, 0u);
         // the group's lanes are one wave's: order its LDS writes before the reads
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        uint32_t total;
-        (void)group_prefix<G>(mine, l, &total);
-        stream_total = total;
+        stream_total = ends;
     }
     // Lane l takes stream positions l, l+G, ... (U = SPHE_DENSITY_UNROLL
     // per iteration, blocks of U*G positions); its row pointer cur advances
-    // through the table as its position passes each row's end.  Positions
-    // past the stream load the particle itself and take nothing.
+    // through the table as its position passes each row's end (the sentinel
+    // row ends every walk).  Blocks wholly inside the stream need no
+    // position checks; in the last, partial block the positions past the
+    // stream load the particle itself and take nothing.
     {
         constexpr int U = SPHE_DENSITY_UNROLL;
         const uint32_t total = stream_total;
         int cur = -1;
-        uint32_t cur_end = 0, cur_off = 0;  // stream end of row cur; slot = t + cur_off inside it
-        for (uint32_t base = 0; base < total; base += U * G) {
+        uint32_t cur_end = 0, cur_off = 0;  // row cur's stream end and slot offset
+        auto block = [&](uint32_t base, auto tail) {
+            constexpr bool TAIL = decltype(tail)::value;
             uint32_t jj[U];
             bool ok[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t t = base + (uint32_t)(u * G + l);
-                ok[u] = t < total;
-                jj[u] = self;
-                if (ok[u]) {
-                    while (t >= cur_end && cur < nrows - 1) {
-                        const uint2 e = tab[++cur];
-                        cur_off = e.x - cur_end;
-                        cur_end += e.y;
-                    }
-                    // the table covers the stream; a position it does not
-                    // cover (never, with consistent counts) takes nothing
-                    ok[u] = t < cur_end;
-                    if (ok[u]) jj[u] = t + cur_off;
+                while (t >= cur_end) {
+                    const uint2 e = tab[++cur];
+                    cur_end = e.x;
+                    cur_off = e.y;
                 }
+                ok[u] = TAIL ? t < total : true;
+                jj[u] = ok[u] ? t + cur_off : self;
             }
             T d2[U];
 #pragma unroll
@@ -791,16 +809,16 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
             // of the block): the block's listed candidates at u' < u (all
             // lanes), then those of lanes below l at u.  Byte-wise sums over
             // the group (each byte stays <= 4 G, no carries): excl = lanes
-            // below l, tot = all lanes; tot * 0x01010100 puts the sum of
-            // bytes 0 .. u-1 into byte u.
+            // below l, tot = all lanes; byte u of place is the sum of tot's
+            // bytes below u plus excl's byte u.  sad_u8 adds tot's four bytes.
             uint32_t tot;
             const uint32_t excl = group_prefix<G>(x, l, &tot) - x;
-            const uint32_t before = tot * 0x01010100u;
-            const uint32_t listed = (tot * 0x01010101u) >> 24;
-            if (cnt + listed <= cap) {
+            const uint32_t place = excl + (byte_runs(tot) << 8);
+            const uint32_t ncnt = __builtin_amdgcn_sad_u8(tot, 0u, cnt);
+            if (ncnt <= cap) {
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    if ((x >> (8 * u)) & 1u) nbl[cnt + ((before >> (8 * u)) & 0xFFu) + ((excl >> (8 * u)) & 0xFFu)] = jj[u];
+                    if ((x >> (8 * u)) & 1u) nbl[cnt + ((place >> (8 * u)) & 0xFFu)] = jj[u];
             } else {
                 if (!claimed) {
                     // first pass over the list capacity: claim maxn pool entries
@@ -815,13 +833,16 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
 #pragma unroll
                 for (int u = 0; u < U; ++u)
                     if ((x >> (8 * u)) & 1u) {
-                        const uint32_t m = cnt + ((before >> (8 * u)) & 0xFFu) + ((excl >> (8 * u)) & 0xFFu);
+                        const uint32_t m = cnt + ((place >> (8 * u)) & 0xFFu);
                         if (m < cap) nbl[m] = jj[u];
                         else if (pb != ~0u && m - cap < cap) D.pool[pb + (m - cap)] = jj[u];
                     }
             }
-            cnt += listed;
-        }
+            cnt = ncnt;
+        };
+        uint32_t base = 0;
+        for (; base + U * G <= total; base += U * G) block(base, std::false_type());
+        if (base < total) block(base, std::true_type());
     }
     group_merge<G>(sum);
     if (l != 0) return;
@@ -1668,11 +1689,12 @@ void sphe_launch_reorder(const DevState& D, hipStream_t st, hipEvent_t e0, hipEv
 template <int G>
 static void launch_density_g(const DevState& D, int block, int remap, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
-    // one row table of W^2 (first slot, length) entries per group, at most
-    // 64 KiB per block (halve the block until it fits)
+    // one row table of W^2 + 1 (first slot, length) entries per group, at
+    // most 64 KiB per block (halve the block until it fits)
     const int W = 2 * D.yz + 1;
-    while (block > 64 && (size_t)(block / G) * W * W * sizeof(uint2) > 65536) block /= 2;
-    const size_t lds = (size_t)(block / G) * W * W * sizeof(uint2);
+    const size_t per_group = (size_t)(W * W + 1) * sizeof(uint2);
+    while (block > 64 && (size_t)(block / G) * per_group > 65536) block /= 2;
+    const size_t lds = (size_t)(block / G) * per_group;
     const int nblk = (int)grid_of(D.P * G, block);
     SPHE_PREC_SWITCH(D.precision,
                      (klaunch_lds(k_density<float, true, G>, nblk, block, lds, st, e0, e1, D, nblk, remap)),
