@@ -85,7 +85,8 @@ __device__ __forceinline__ float d_fmin(float a, float b) { return __builtin_fmi
 __device__ __forceinline__ double d_fmin(double a, double b) { return __builtin_fmin(a, b); }
 
 // A floored cell coordinate clamped to [0, nmax]; negative and NaN give 0
-// (max(NaN, 0) = 0), branch-free.
+// (max(NaN, 0) = 0), branch-free.  nmax is exact in T: the layout keeps grid
+// and sub-cell extents below 2^24.
 template <typename T>
 __device__ __forceinline__ int cidx(T f, int nmax)
 {

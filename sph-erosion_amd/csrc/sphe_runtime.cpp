@@ -371,6 +371,10 @@ int build_layout(sphe_ctx* c)
         s.p_cap = round_up((int64_t)s.ids.size(), SPHE_WAVE);
         P += s.p_cap;
         grid_dims(s.prm, s.dims, c->yz);
+        // the neighbour passes clamp cell and x sub-cell indices (4 per cell
+        // at most) in FP32, exact below 2^24
+        if ((int64_t)s.dims[0] * 4 > (int64_t)1 << 24 || s.dims[1] > 1 << 24 || s.dims[2] > 1 << 24)
+            return fail(c, SPHE_ERANGE, "grid too long (%d x %d x %d cells)", s.dims[0], s.dims[1], s.dims[2]);
         s.ncell = (int64_t)s.dims[0] * s.dims[1] * s.dims[2];
         s.cell_off = C;
         C += s.ncell + 1;
