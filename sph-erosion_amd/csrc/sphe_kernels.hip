@@ -433,9 +433,12 @@ __global__ __launch_bounds__(256) void k_hash_count(DevState D)
 #define SCAN_TILE (256 * SCAN_ITEMS)
 static_assert(SCAN_TILE == (1 << SPHE_SCAN_TILE_LOG2), "scan tile");
 
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uint32_t* lds)
+// Exclusive scan of v over a 256-thread block, with its total, and the
+// block total of a second value w: one LDS exchange (lds: 8 words).
+__device__ __forceinline__ uint32_t block_scan_sum(uint32_t v, uint32_t w, uint32_t* total, uint32_t* wtotal,
+                                                   uint32_t* lds)
 {
-    // wave-inclusive scan with shuffles, then a scan of the 4 wave totals
+    // wave-inclusive scan and wave sum with shuffles, then the 4 wave totals
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t x = v;
 #pragma unroll
@@ -443,17 +446,22 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
         uint32_t y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
     }
+    uint32_t r = w;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) r += __shfl_xor(r, o, 64);
     if (lane == 63) lds[wave] = x;
+    if (lane == 0) lds[4 + wave] = r;
     __syncthreads();
-    uint32_t base = 0, tot = 0;
-    const int nw = blockDim.x >> 6;
-    for (int w = 0; w < nw; ++w) {
-        uint32_t t = lds[w];
-        if (w < wave) base += t;
+    uint32_t base = 0, tot = 0, wt = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t t = lds[k];
+        if (k < wave) base += t;
         tot += t;
+        wt += lds[4 + k];
     }
-    __syncthreads();
     *total = tot;
+    *wtotal = wt;
     return base + x - v;
 }
 
@@ -504,8 +512,8 @@ __device__ __forceinline__ void scan_reduce_tile(const uint32_t* __restrict__ in
     uint32_t s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) s += v[i];
-    uint32_t tot;
-    block_exclusive_scan(s, &tot, lds);
+    uint32_t tot, none;
+    (void)block_scan_sum(0u, s, &none, &tot, lds);
     if (threadIdx.x == 0) partial[tile] = tot;
 }
 
@@ -513,7 +521,7 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
                                                      uint32_t* __restrict__ partial,
                                                      const uint32_t* __restrict__ occ)
 {
-    __shared__ uint32_t lds[4];
+    __shared__ uint32_t lds[8];
     scan_reduce_tile(in, n, partial, occ, (int)blockIdx.x, lds);
 }
 
@@ -538,16 +546,23 @@ __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, u
     } else {
         load_run(cnt, base, n, v);
     }
+    // the totals of the tiles before this one, eight loads in flight per thread
+    uint32_t prior = 0;
+    for (uint32_t q0 = threadIdx.x; q0 < blockIdx.x; q0 += 8 * blockDim.x) {
+        uint32_t pv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t q = q0 + (uint32_t)k * blockDim.x;
+            pv[k] = q < blockIdx.x ? partial[q] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) prior += pv[k];
+    }
     uint32_t s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) s += v[i];
-    uint32_t tot;
-    uint32_t run = block_exclusive_scan(s, &tot, lds);
-    // the totals of the tiles before this one
-    uint32_t prior = 0;
-    for (uint32_t q = threadIdx.x; q < blockIdx.x; q += blockDim.x) prior += partial[q];
-    uint32_t ptot;
-    (void)block_exclusive_scan(prior, &ptot, lds + 4);
+    uint32_t tot, ptot;
+    uint32_t run = block_scan_sum(s, prior, &tot, &ptot, lds);
     run += ptot;
     uint32_t z[SCAN_ITEMS];
 #pragma unroll
@@ -559,7 +574,7 @@ __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, u
     }
     store_run(start, base, n, v);
     if (zero_counts && !empty) store_run(cnt, base, n, z);
-    // every thread read the flag before the barriers of the scans above
+    // every thread read the flag before the barrier of the scan above
     if (occ && !empty && threadIdx.x == 0) occ[blockIdx.x] = 0u;
 }
 
@@ -596,6 +611,13 @@ __global__ __launch_bounds__(256) void k_reorder(DevState D)
     if (d >= D.P) return;
     const int sl = D.sub_log2;
     const uint64_t kid = D.tmp_kid[d];
+    const int32_t src = D.tmp_src[d];
+    // the particle's state, gathered while its rank is found (from a slot
+    // clamped into range: a bad src is reported below and never stored)
+    const uint32_t sc = (uint32_t)min(max(src, 0), (int32_t)(D.P - 1));
+    const V4<T> pv = at((const V4<T>*)D.u_pos, sc);
+    const V4<T> vv = at((const V4<T>*)D.u_vel, sc);
+    const int64_t sed = D.u_sed[sc];
     const int32_t fk = (int32_t)(kid >> 32);
     const int32_t c = fk >> sl;
     if (fk < 0 || (int64_t)c >= D.C) {
@@ -605,26 +627,31 @@ __global__ __launch_bounds__(256) void k_reorder(DevState D)
     const uint32_t s0 = D.start[c];
     const uint32_t s1 = (int64_t)D.start[c + 1] <= D.P ? D.start[c + 1] : (uint32_t)D.P;
     const int32_t my = (int32_t)((uint32_t)kid ^ 0x80000000u);
-    const int32_t src = D.tmp_src[d];
     // one count field per slab boundary: a particle in slab k adds 1 to the
     // fields of the boundaries s > k
     const uint32_t fields = sl == 2 ? 0x100401u : (sl == 1 ? 1u : 0u);
     const uint32_t smask = (1u << sl) - 1u;
     uint32_t rank = 0, below = 0;
-    for (uint32_t j = s0; j < s1; ++j) {
-        const uint64_t o = D.tmp_kid[j];
+    auto take = [&](uint32_t j, uint64_t o) {
         rank += (o < kid || (o == kid && D.tmp_src[j] < src)) ? 1u : 0u;
         below += fields & (~0u << (10u * ((uint32_t)(o >> 32) & smask)));
+    };
+    uint32_t j = s0;
+    for (; j + 1 < s1; j += 2) {  // two loads in flight
+        const uint64_t o0 = D.tmp_kid[j], o1 = D.tmp_kid[j + 1];
+        take(j, o0);
+        take(j + 1, o1);
     }
+    if (j < s1) take(j, D.tmp_kid[j]);
     const uint32_t f = s0 + rank;
     if ((int64_t)f >= D.P || src < 0 || (int64_t)src >= D.P) {
         atomicOr(D.status, 2u);
         return;
     }
     if (rank == 0) D.subw[c] = s1 - s0 > 1023u ? 0x80000000u : below;
-    ((V4<T>*)D.s_pos)[f] = ((const V4<T>*)D.u_pos)[src];
-    ((V4<T>*)D.s_vel)[f] = ((const V4<T>*)D.u_vel)[src];
-    D.s_sed[f] = D.u_sed[src];
+    ((V4<T>*)D.s_pos)[f] = pv;
+    ((V4<T>*)D.s_vel)[f] = vv;
+    D.s_sed[f] = sed;
     D.s_id[f] = my;
 }
 
@@ -1356,7 +1383,7 @@ __global__ __launch_bounds__(256) void k_heights(DevState D)
 // eroded, before this step's force pass samples them.
 __global__ __launch_bounds__(256) void k_scan_reduce_heights(DevState D, int nscan)
 {
-    __shared__ uint32_t lds[4];
+    __shared__ uint32_t lds[8];
     if ((int)blockIdx.x >= nscan) {  // block-uniform: no barrier is skipped by part of a block
         const int64_t t0 = ((int64_t)(blockIdx.x - nscan) * blockDim.x + threadIdx.x) * 4;
         if (t0 < D.TC) heights_quad(D, t0);
