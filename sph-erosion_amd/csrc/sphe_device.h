@@ -76,6 +76,10 @@ struct DevState {
     // in the tile is taken, cleared by the scan that consumes the tile; the
     // scan reads the counts of flagged tiles only (the rest are zero)
     uint32_t* tocc;
+    // one word per scan tile of start: the value every start of the tile
+    // holds when the tile was last written empty, ~0 otherwise; the scan
+    // leaves an empty tile unwritten when that value is still right
+    uint32_t* tfill;
     // cells are split into 2^sub_log2 slabs along x (sub_log2 <= 2); subw
     // holds each cell's slab offsets (10 bits per inner boundary; bit 31:
     // more than 1023 particles, no offsets), rewritten every step for the

@@ -530,15 +530,19 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
 // particle's scatter reset its own cell); the optional single word zero is
 // reset too.  occ (optional): the tile flags; an unflagged tile's counts are
 // not read (all zero), and each block clears its tile's flag once read.
+// fill (optional, with occ): per tile, the value all of an empty tile's
+// starts hold since the tile was last written empty (~0: not known); an
+// empty tile whose prefix equals it is not written again.
 __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, uint32_t* __restrict__ start,
                                                    int64_t n, const uint32_t* __restrict__ partial,
                                                    int zero_counts, unsigned long long* __restrict__ zero,
-                                                   uint32_t* __restrict__ occ)
+                                                   uint32_t* __restrict__ occ, uint32_t* __restrict__ fill)
 {
     if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0ull;
     __shared__ uint32_t lds[8];
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
     const bool empty = occ && occ[blockIdx.x] == 0u;  // block-uniform
+    const uint32_t filled = fill ? fill[blockIdx.x] : ~0u;
     uint32_t v[SCAN_ITEMS];
     if (empty) {
 #pragma unroll
@@ -572,10 +576,19 @@ __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, u
         run += c;
         z[i] = 0u;
     }
-    store_run(start, base, n, v);
+    // an empty tile's starts all equal ptot (its prefix): unwritten when they
+    // already hold it
+    if (!(empty && filled == ptot)) store_run(start, base, n, v);
     if (zero_counts && !empty) store_run(cnt, base, n, z);
-    // every thread read the flag before the barrier of the scan above
-    if (occ && !empty && threadIdx.x == 0) occ[blockIdx.x] = 0u;
+    // every thread read the flag and the fill word before the barrier of the
+    // scan above
+    if (threadIdx.x == 0) {
+        if (occ && !empty) occ[blockIdx.x] = 0u;
+        if (fill) {
+            const uint32_t nf = empty ? ptot : ~0u;
+            if (nf != filled) fill[blockIdx.x] = nf;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_scatter(DevState D)
@@ -1690,13 +1703,14 @@ void sphe_launch_hash(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent
 }
 
 void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, bool zero_counts,
-                      unsigned long long* zero, uint32_t* occ, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+                      unsigned long long* zero, uint32_t* occ, uint32_t* fill, hipStream_t st, hipEvent_t e0,
+                      hipEvent_t e1)
 {
     const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     klaunch(k_scan_reduce, (unsigned)nb, 256, st, e0, nullptr, (const uint32_t*)cnt, n, partial,
             (const uint32_t*)occ);
     klaunch(k_scan_down, (unsigned)nb, 256, st, nullptr, e1, cnt, start, n, (const uint32_t*)partial,
-            zero_counts ? 1 : 0, zero, occ);
+            zero_counts ? 1 : 0, zero, occ, occ ? fill : nullptr);
 }
 
 int64_t sphe_scan_partials(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
@@ -1771,7 +1785,7 @@ void sphe_launch_scan_heights(const DevState& D, hipStream_t st, hipEvent_t e0, 
     const int64_t nh = (int64_t)grid_of(D.TC / 4, 256);  // TC: a multiple of 64
     klaunch(k_scan_reduce_heights, (unsigned)(nb + nh), 256, st, e0, nullptr, D, (int)nb);
     klaunch(k_scan_down, (unsigned)nb, 256, st, nullptr, e1, D.cnt, D.start, n, (const uint32_t*)D.partial, 0,
-            D.ptop, D.tocc);
+            D.ptop, D.tocc, D.tfill);
 }
 
 void sphe_launch_heights(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
@@ -1784,7 +1798,7 @@ void sphe_launch_two_phase(const DevState& D, hipStream_t st, hipEvent_t e0, hip
 {
     if (D.TC == 0 || D.P == 0) return;
     klaunch(k_tcount, grid_of(D.P, 256), 256, st, e0, nullptr, D);
-    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, true, nullptr, nullptr, st);
+    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, true, nullptr, nullptr, nullptr, st);
     k_tscatter<<<grid_of(D.P, 256), 256, 0, st>>>(D);
     klaunch(k_tapply, grid_of(D.TC, 256), 256, st, nullptr, e1, D);
 }

@@ -417,6 +417,7 @@ int build_layout(sphe_ctx* c)
     D.npartial = sphe_scan_partials(C + 1);
     DALLOC(D.partial, (size_t)D.npartial * 4);
     DALLOC(D.tocc, (size_t)D.npartial * 4);
+    DALLOC(D.tfill, (size_t)D.npartial * 4);
     DALLOC(D.vol, (size_t)TC * 8);
     DALLOC(D.height, (size_t)TC * 4);
     DALLOC(D.dq, Pn * 8);
@@ -458,6 +459,7 @@ int build_layout(sphe_ctx* c)
     HIPCHK(c, hipMemcpyAsync(tcs, htcs.data(), htcs.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(D.cnt, 0, (size_t)(C + 1) * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(D.tocc, 0, (size_t)D.npartial * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(D.tfill, 0xFF, (size_t)D.npartial * 4, c->stream));  // no tile known filled
     HIPCHK(c, hipMemsetAsync(D.subw, 0, (size_t)(C + 1) * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(D.tcnt, 0, (size_t)(TC + 1) * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(D.status, 0, 64, c->stream));
@@ -710,7 +712,8 @@ void launch_sort(sphe_ctx* c)
         // with erosion the terrain heights of the last force pass are computed
         // in the scan's first launch (this step's force pass samples them)
         if (any_erosion(c, -1) && D.TC > 0) sphe_launch_scan_heights(D, c->stream, t.a, t.b);
-        else sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, false, D.ptop, D.tocc, c->stream, t.a, t.b);
+        else sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, false, D.ptop, D.tocc, D.tfill, c->stream, t.a,
+                              t.b);
     }
     {
         Timed t(c, "scatter");
