@@ -79,6 +79,13 @@ __device__ __forceinline__ V at(const V* __restrict__ base, uint32_t j)
     return *(const V*)((const char*)base + j * (uint32_t)sizeof(V));
 }
 
+// The same at a byte offset (j * sizeof(V), precomputed).
+template <typename V>
+__device__ __forceinline__ V at_byte(const V* __restrict__ base, uint32_t off)
+{
+    return *(const V*)((const char*)base + off);
+}
+
 __device__ __forceinline__ float d_fmax(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ double d_fmax(double a, double b) { return __builtin_fmax(a, b); }
 __device__ __forceinline__ float d_fmin(float a, float b) { return __builtin_fminf(a, b); }
@@ -724,7 +731,9 @@ __device__ __forceinline__ uint32_t group_prefix<4>(uint32_t x, int l, uint32_t*
 // density sum.
 // The group writes one neighbour list per particle (self left out) in
 // canonical order, nbl[slot * maxn + m], m < maxn, count in nbc[slot]; a
-// byte-wise prefix sum over the group places each lane's entries.  The
+// byte-wise prefix sum over the group places each lane's entries.  List
+// entries are byte offsets of the neighbours' slots in the per-slot vectors
+// of V4<T> (slot * sizeof(V4<T>), below 2^32 by the layout's limits).  The
 // group then merges the partial sums and lane 0 writes rho and
 // (p / rho^2, m / rho).
 template <typename T, bool COMP, int G>
@@ -744,7 +753,8 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     const T h2 = K.h2, thr = K.thr;
     const uint32_t cap = (uint32_t)D.maxn;
     uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn;
-    const uint32_t self = (uint32_t)slot;
+    constexpr uint32_t VB = (uint32_t)sizeof(V4<T>);  // bytes per slot of a V4<T> vector
+    const uint32_t self = (uint32_t)slot * VB;          // as a byte offset
     const int gfirst = (int)(threadIdx.x & (SPHE_WAVE - 1)) & ~(G - 1);  // the group's first lane in the wave
     uint32_t cnt = 0;      // the same in every lane of the group
     uint32_t pb = ~0u;     // the group's overflow chunk in the pool (none yet)
@@ -752,12 +762,12 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     uint32_t stream_total = 0;
     // The particle's clipped rows as one stream of candidates, t = 0 ..
     // total-1 in canonical order.  Lane l computes rows l, l+G, ... (the
-    // group takes G rows per round) and writes (stream end, slot offset) to
-    // the group's row table in LDS (dynamic: blockDim / G tables of W^2 + 1
-    // entries): positions t of row r lie below its end and hold slot t +
-    // offset.  A group prefix sum per round gives the stream ends.  Entry
-    // W^2 is a sentinel row that ends at 2^32 - 1, so a walk along the table
-    // stops by itself.
+    // group takes G rows per round) and writes (stream end, slot offset),
+    // both in bytes (times VB), to the group's row table in LDS (dynamic:
+    // blockDim / G tables of W^2 + 1 entries): positions t of row r lie below
+    // its end and hold slot t + offset.  A group prefix sum per round gives
+    // the stream ends.  Entry W^2 is a sentinel row that ends at 2^32 - 1,
+    // so a walk along the table stops by itself.
     const int yz = D.yz, W = 2 * yz + 1, nrows = W * W;
     extern __shared__ uint2 rowtab[];
     uint2* __restrict__ tab = rowtab + (threadIdx.x / G) * (nrows + 1);
@@ -779,7 +789,7 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
             }
             uint32_t rtot;
             const uint32_t end = ends + group_prefix<G>(len, l, &rtot);
-            if (r < nrows) tab[r] = make_uint2(end, j0 - (end - len));
+            if (r < nrows) tab[r] = make_uint2(end * VB, (j0 - (end - len)) * VB);
             ends += rtot;
             dy += G;
             roff += (uint32_t)(G * nx);
@@ -799,9 +809,10 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     // Lane l takes stream positions l, l+G, ... (U = SPHE_DENSITY_UNROLL
     // per iteration, blocks of U*G positions); its row pointer cur advances
     // through the table as its position passes each row's end (the sentinel
-    // row ends every walk).  Blocks wholly inside the stream need no
-    // position checks; in the last, partial block the positions past the
-    // stream load the particle itself and take nothing.
+    // row ends every walk).  Positions, row ends and slots are in bytes
+    // (times VB).  Blocks wholly inside the stream need no position checks;
+    // in the last, partial block the positions past the stream load the
+    // particle itself and take nothing.
     {
         constexpr int U = SPHE_DENSITY_UNROLL;
         const uint32_t total = stream_total;
@@ -814,24 +825,27 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t t = base + (uint32_t)(u * G + l);
-                while (t >= cur_end) {
+                const uint32_t tb = t * VB;
+                while (tb >= cur_end) {
                     const uint2 e = tab[++cur];
                     cur_end = e.x;
                     cur_off = e.y;
                 }
                 ok[u] = TAIL ? t < total : true;
-                jj[u] = ok[u] ? t + cur_off : self;
+                jj[u] = ok[u] ? tb + cur_off : self;
             }
             T d2[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const V4<T> p = at(pos, jj[u]);
+                const V4<T> p = at_byte(pos, jj[u]);
                 const T dx = pi.x - p.x, dy = pi.y - p.y, dz = pi.z - p.z;
                 d2[u] = (dx * dx + dy * dy) + dz * dz;
             }
             // The lane's U terms (0 for a candidate outside the radius or
             // past the stream) are added in plain arithmetic, u = 0 first,
-            // and the block's partial enters the lane's sum.
+            // and the block's partial enters the lane's sum.  Outside the
+            // radius the term is 0 without a test: d2 > thr >= h2 gives
+            // q = 0 (h2 = fl(h h) <= thr since fl(sqrt(fl(h h))) = h).
             // byte u of x: 1 if this lane lists its u-th candidate
             uint32_t x = 0;
             T part = T(0);
@@ -840,7 +854,7 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
                 const bool in = ok[u] & (d2[u] <= thr);
                 T q = h2 - d2[u];
                 q = q > T(0) ? q : T(0);
-                const T w = in ? (q * q) * q : T(0);
+                const T w = ok[u] ? (q * q) * q : T(0);
                 part = u ? part + w : w;
                 // the force pass adds the particle's own term first: self is not listed
                 x |= (uint32_t)(in & (jj[u] != self)) << (8 * u);
@@ -1181,7 +1195,8 @@ __device__ __forceinline__ int64_t erode_exchange(const SceneDev& S, const Scene
 }
 
 // Lane l's share of the force sums from list entries l, l+G, l+2G, ... of
-// a particle's neighbour sequence (fetch(k) = its k-th neighbour): the lanes
+// a particle's neighbour sequence (fetch(k) = the byte offset of its k-th
+// neighbour's slot in the V4<T> vectors): the lanes
 // of a particle differ by one neighbour at most.  Two pairs per iteration
 // (eight gathers in flight), then one pair, then a single.
 template <typename T, bool COMP, int G, typename F>
@@ -1194,8 +1209,8 @@ __device__ __forceinline__ void force_list(const DevState& D, const SceneConsts<
     uint32_t k = (uint32_t)l;
     for (; k + 3 * G < cnt; k += 4 * G) {
         const uint32_t ja = fetch(k), jb = fetch(k + G), jc = fetch(k + 2 * G), jd = fetch(k + 3 * G);
-        const V4<T> pa = at(posa, ja), va = at(vel, ja), pb = at(posa, jb), vb = at(vel, jb);
-        const V4<T> pc = at(posa, jc), vc = at(vel, jc), pd = at(posa, jd), vd = at(vel, jd);
+        const V4<T> pa = at_byte(posa, ja), va = at_byte(vel, ja), pb = at_byte(posa, jb), vb = at_byte(vel, jb);
+        const V4<T> pc = at_byte(posa, jc), vc = at_byte(vel, jc), pd = at_byte(posa, jd), vd = at_byte(vel, jd);
         {
             const T ax = pi.x - pa.x, ay = pi.y - pa.y, az = pi.z - pa.z;
             const T a2 = (ax * ax + ay * ay) + az * az;
@@ -1216,10 +1231,10 @@ __device__ __forceinline__ void force_list(const DevState& D, const SceneConsts<
     for (; k + G < cnt; k += 2 * G) {
         const uint32_t ja = fetch(k);
         const uint32_t jb = fetch(k + G);
-        const V4<T> pa = at(posa, ja);
-        const V4<T> va = at(vel, ja);
-        const V4<T> pb = at(posa, jb);
-        const V4<T> vb = at(vel, jb);
+        const V4<T> pa = at_byte(posa, ja);
+        const V4<T> va = at_byte(vel, ja);
+        const V4<T> pb = at_byte(posa, jb);
+        const V4<T> vb = at_byte(vel, jb);
         const T ax = pi.x - pa.x, ay = pi.y - pa.y, az = pi.z - pa.z;
         const T a2 = (ax * ax + ay * ay) + az * az;
         const T bx = pi.x - pb.x, by = pi.y - pb.y, bz = pi.z - pb.z;
@@ -1229,8 +1244,8 @@ __device__ __forceinline__ void force_list(const DevState& D, const SceneConsts<
     }
     if (k < cnt) {
         const uint32_t j = fetch(k);
-        const V4<T> pj = at(posa, j);
-        const V4<T> vj = at(vel, j);
+        const V4<T> pj = at_byte(posa, j);
+        const V4<T> vj = at_byte(vel, j);
         const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const T d2 = (dx * dx + dy * dy) + dz * dz;
         acc.add(pair_terms<T>(K, dx, dy, dz, d2, a_i, pj.w, vj.w, vi, vj));

@@ -382,9 +382,11 @@ int build_layout(sphe_ctx* c)
         s.tcap = round_up((int64_t)s.tnx * s.tnz, SPHE_WAVE);
         TC += s.tcap;
     }
-    // per-slot vectors are addressed with 32-bit byte offsets: each stays under 4 GiB
+    // per-slot vectors are addressed with 32-bit byte offsets: each stays under
+    // 4 GiB, with room for the density pass's positions a block past the end
     const int64_t vec_bytes = 4 * (int64_t)real_size(c->precision);
-    if (P > INT32_MAX / 2 || C > INT32_MAX / 2 || TC > INT32_MAX / 2 || P * vec_bytes > (int64_t)UINT32_MAX)
+    if (P > INT32_MAX / 2 || C > INT32_MAX / 2 || TC > INT32_MAX / 2 ||
+        (P + 64) * vec_bytes > (int64_t)UINT32_MAX)
         return fail(c, SPHE_ERANGE, "context too large (slots %lld, cells %lld)", (long long)P, (long long)C);
     DevState& D = c->D;
     D.P = P;

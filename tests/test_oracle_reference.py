@@ -186,6 +186,20 @@ def test_predicate_threshold_exhaustive_walk():
     assert np.float32(O.pred_threshold(np.float32(0.0457))).view(np.uint32) == 0x3B08DF0C
 
 
+def test_h2_never_above_threshold():
+    """fl32(h*h) <= T(h) for every h: sqrt of a correctly rounded square rounds
+    back to h.  The density kernels rely on it: a candidate outside the
+    radius (d2 > T) has h2 - d2 < 0, so its clamped term is 0 without a test."""
+    rng = np.random.default_rng(5)
+    hs = np.concatenate([np.float32(2.0) ** rng.uniform(-14, 4, 20000),
+                         np.float32(2.0) ** np.arange(-14, 5)]).astype(np.float32)
+    hs = np.concatenate([hs, np.nextafter(hs, np.float32(0)), np.nextafter(hs, np.float32(np.inf))])
+    h2 = hs * hs  # float32 products, rounded to nearest
+    T = np.array([O.pred_threshold(h) for h in hs], dtype=np.float32)
+    assert np.all(h2 <= T)
+    assert np.all(np.sqrt(h2) == hs)
+
+
 def test_fp32_root_is_within_ulps_of_sqrt():
     """The FP32 kernels' r = |d| and 1/r (three Newton steps for 1/sqrt from
     an integer estimate) against correctly rounded sqrt and 1/sqrt: within
