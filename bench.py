@@ -211,8 +211,9 @@ def main():
     sids = [ctx.add_synthetic(s) for s in scs]
     if not a.no_tune:
         blk, rm = ctx.autotune(5)
-        log(f"bench: autotuned block={blk} xcd_remap={rm}")
+        log(f"bench: autotuned block={blk} force_block={ctx.get_force_block()} xcd_remap={rm}")
     blk, rm = ctx.get_tuning()
+    fblk = ctx.get_force_block()
     if a.trace:
         # ms/step and neighbour counts along the trajectory, 50-step chunks
         for c in range(0, a.trace, 50):
@@ -275,7 +276,7 @@ def main():
                ms_per_step=1e3 * elapsed / a.steps, higher_is_better=True, scaling="weak",
                vs_baseline=None, dtype="fp32", data="synthetic",
                config=dict(workload=f"{a.workload}: {desc}", particles=N, scenes=len(scs),
-                           precision="fp32 compensated (Kahan)", block=blk, density_xcd_run=rm,
+                           precision="fp32 compensated (Kahan)", block=blk, force_block=fblk, density_xcd_run=rm,
                            x_slabs=ctx.get_subcells(), yz_split=ctx.get_yz_split()),
                roofline=roofline)
     out["value_kernel_by_kernel"] = N * a.steps / prof_elapsed
@@ -310,6 +311,7 @@ def main():
         for s in scs:
             c64.add_synthetic(s)
         c64.set_tuning(blk, rm)
+        c64.set_force_block(fblk)
         c64.step(a.warmup)
         c64.sync()
         t = time.perf_counter()

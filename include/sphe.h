@@ -256,11 +256,17 @@ int sphe_kernel_bytes(sphe_ctx* ctx, const char* kernel, double* bytes);
  * xcd_remap consecutive blocks (default 128: neighbouring cells share an
  * XCD's L2; 0 or 1: plain round-robin order).  The force pass always uses
  * the plain order, which keeps the XCDs balanced when neighbour counts vary
- * across the domain.  sphe_autotune times the step for block sizes
- * {64, 128, 256, 512} with the run as set and keeps the fastest.  Neither
- * parameter changes results. */
+ * across the domain.  sphe_set_tuning sets the block size of both neighbour
+ * passes; sphe_set_force_block then gives the force pass its own (0: the
+ * density pass's), and sphe_get_force_block reads the one in effect.
+ * sphe_autotune runs the step with block sizes {64, 128, 256, 512}, times
+ * the density and force passes of each trial with the profiling events, and
+ * keeps each pass's fastest, with the run as set.  No parameter here
+ * changes results.  No counterpart in the reference (a serial CPU loop). */
 int sphe_set_tuning(sphe_ctx* ctx, int block, int xcd_remap);
 int sphe_get_tuning(const sphe_ctx* ctx, int* block, int* xcd_remap);
+int sphe_set_force_block(sphe_ctx* ctx, int block);
+int sphe_get_force_block(const sphe_ctx* ctx, int* block);
 /* Lanes per particle in the density and force passes (1, 2, 4 or 8;
  * default 4).  Lane l takes positions l, l+G, ... of each stencil row and the
  * partial sums are merged in a fixed xor tree, so results are deterministic

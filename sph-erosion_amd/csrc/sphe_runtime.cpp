@@ -89,6 +89,7 @@ struct sphe_ctx {
     std::map<std::pair<int, int>, std::vector<float>> views;  // sphe_buffer snapshots by (scene, buffer)
 
     int block = 128;
+    int fblock = 0;  // block size of the force pass (0: the density pass's)
     int remap = 128;  // density: runs of 128 consecutive blocks per XCD (L2 locality); force: plain order
     int maxn = 256;  // neighbour-list capacity per particle (grown when a step overflows it)
     int lanes = 4;   // lanes per particle in the density and force passes
@@ -699,6 +700,9 @@ struct Timed {
     }
 };
 
+// block size of the force pass: its own when set, else the density pass's
+int force_block(const sphe_ctx* c) { return c->fblock ? c->fblock : c->block; }
+
 bool any_erosion(const sphe_ctx* c, int mode)
 {
     for (const Scene& s : c->scenes)
@@ -737,7 +741,7 @@ void launch_step(sphe_ctx* c)
     }
     {
         Timed t(c, "force");
-        sphe_launch_force(D, c->block, 1, c->lanes, c->stream, t.a, t.b);  // plain XCD order: balance
+        sphe_launch_force(D, force_block(c), 1, c->lanes, c->stream, t.a, t.b);  // plain XCD order: balance
     }
     if (any_erosion(c, SPHE_EROSION_TWO_PHASE)) {
         Timed t(c, "two_phase");
@@ -748,7 +752,8 @@ void launch_step(sphe_ctx* c)
 std::string plan_signature(const sphe_ctx* c)
 {
     char b[256];
-    snprintf(b, sizeof(b), "%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%p/%d/%d/%d", c->precision, c->block, c->remap,
+    snprintf(b, sizeof(b), "%d/%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%p/%d/%d/%d", c->precision, c->block,
+             force_block(c), c->remap,
              c->lanes, (long long)c->D.P, (long long)c->D.C, (long long)c->D.TC, (int)any_erosion(c, -1),
              (int)any_erosion(c, SPHE_EROSION_TWO_PHASE), (const void*)c->D.u_pos, (const void*)c->D.nbl,
              (const void*)c->D.pool, c->D.maxn, c->D.sub_log2, c->D.yz);
@@ -1681,7 +1686,24 @@ int sphe_set_tuning(sphe_ctx* c, int block, int remap)
         return fail(c, SPHE_EARG, "block must be 64, 128, 256 or 512");
     if (remap < 0) return fail(c, SPHE_EARG, "xcd_remap (blocks per XCD run) must be >= 0");
     c->block = block;
+    c->fblock = 0;
     c->remap = remap;
+    return SPHE_OK;
+}
+
+int sphe_set_force_block(sphe_ctx* c, int block)
+{
+    if (!c) return SPHE_EARG;
+    if (block != 0 && block != 64 && block != 128 && block != 256 && block != 512)
+        return fail(c, SPHE_EARG, "force block must be 0, 64, 128, 256 or 512");
+    c->fblock = block;
+    return SPHE_OK;
+}
+
+int sphe_get_force_block(const sphe_ctx* c, int* block)
+{
+    if (!c) return SPHE_EARG;
+    if (block) *block = force_block(c);
     return SPHE_OK;
 }
 
@@ -1818,60 +1840,78 @@ int sphe_autotune(sphe_ctx* c, int steps_per_trial)
     if (!c) return SPHE_EARG;
     if (steps_per_trial < 1) steps_per_trial = 1;
     static std::mutex mu;
-    static std::map<std::string, std::pair<int, int>> cache;
+    struct Tuned {
+        int block, fblock, remap;
+    };
+    static std::map<std::string, Tuned> cache;
     int r = prepare(c);
     if (r) return r;
     hipDeviceProp_t prop;
     HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
-    std::string key = std::string(prop.gcnArchName) + "|" + plan_signature(c).substr(0, 64);
+    // what the choice depends on: the device, the kernels' variants and the sizes (not the tuning)
+    std::string key = std::string(prop.gcnArchName) + "|" + std::to_string(c->precision) + "/" +
+                      std::to_string(c->lanes) + "/" + std::to_string(c->remap) + "/" + std::to_string(c->D.P) + "/" +
+                      std::to_string(c->D.C) + "/" + std::to_string(c->D.maxn) + "/" + std::to_string(c->D.sub_log2) +
+                      "/" + std::to_string(c->D.yz) + "/" + std::to_string((int)any_erosion(c, -1));
     for (const Scene& s : c->scenes) key += "|" + std::to_string(s.ids.size()) + ":" + std::to_string(s.prm.h);
     {
         std::lock_guard<std::mutex> g(mu);
         auto it = cache.find(key);
         if (it != cache.end()) {
-            c->block = it->second.first;
-            c->remap = it->second.second;
+            c->block = it->second.block;
+            c->fblock = it->second.fblock;
+            c->remap = it->second.remap;
             return SPHE_OK;
         }
     }
     r = sync_to_host(c);
     if (r) return r;
+    // each trial times the density and force passes with the profiling
+    // events (direct launches), and each pass keeps its own fastest block
+    r = collect_profile(c);
+    if (r) return r;
     const bool prof = c->profiling;
-    c->profiling = false;
-    int best_b = c->block, best_r = c->remap;
-    double best = 1e30;
-    hipEvent_t e0 = get_event(c), e1 = get_event(c);
-    // the block size only: the best XCD run length depends on how uneven the
+    const auto saved_prof = c->prof;
+    int best_b = c->block, best_fb = force_block(c);
+    double best_d = 1e30, best_f = 1e30;
+    // the block sizes only: the best XCD run length depends on how uneven the
     // state is (long runs win on uniform states and lose by up to a fifth on
     // pile-ups), which a trial on the current state cannot foresee; the run
     // stays as set (default 128, near the best on both kinds of state)
-    for (int b : {64, 128, 256, 512})
-        for (int rm : {c->remap}) {
-            c->block = b;
-            c->remap = rm;
-            c->device_valid = false;  // every trial starts from the same state
-            r = do_steps(c, 1);
-            if (r) return r;
-            HIPCHK(c, hipEventRecord(e0, c->stream));
+    for (int b : {64, 128, 256, 512}) {
+        c->block = b;
+        c->fblock = b;
+        c->device_valid = false;  // every trial starts from the same state
+        c->profiling = false;
+        r = do_steps(c, 1);
+        if (r == SPHE_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = fail(c, SPHE_EHIP, "autotune sync");
+        if (r == SPHE_OK) {
+            c->prof.clear();
+            c->profiling = true;
             r = do_steps(c, steps_per_trial);
-            if (r) return r;
-            HIPCHK(c, hipEventRecord(e1, c->stream));
-            HIPCHK(c, hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) { best = ms; best_b = b; best_r = rm; }
+            if (r == SPHE_OK) r = collect_profile(c);
+            c->profiling = false;
         }
-    c->event_pool.push_back(e0);
-    c->event_pool.push_back(e1);
+        if (r) {
+            c->prof = saved_prof;
+            c->profiling = prof;
+            return r;
+        }
+        const auto d = c->prof.find("density"), f = c->prof.find("force");
+        const double dm = d != c->prof.end() ? d->second.first : 0.0, fm = f != c->prof.end() ? f->second.first : 0.0;
+        if (dm < best_d) { best_d = dm; best_b = b; }
+        if (fm < best_f) { best_f = fm; best_fb = b; }
+    }
+    c->prof = saved_prof;
     c->block = best_b;
-    c->remap = best_r;
+    c->fblock = best_fb;
     c->profiling = prof;
     c->host_valid = true;     // host staging still holds the pre-tuning state
     c->device_valid = false;  // restore it on the next use
     c->stepped = false;
     {
         std::lock_guard<std::mutex> g(mu);
-        cache[key] = {best_b, best_r};
+        cache[key] = {c->block, c->fblock, c->remap};
     }
     return SPHE_OK;
 }

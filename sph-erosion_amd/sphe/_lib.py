@@ -96,6 +96,8 @@ SIGNATURES = {
     "sphe_kernel_bytes": (I, [P, C.c_char_p, C.POINTER(C.c_double)]),
     "sphe_set_tuning": (I, [P, I, I]),
     "sphe_get_tuning": (I, [P, C.POINTER(I), C.POINTER(I)]),
+    "sphe_set_force_block": (I, [P, I]),
+    "sphe_get_force_block": (I, [P, C.POINTER(I)]),
     "sphe_autotune": (I, [P, I]),
     "sphe_set_lanes": (I, [P, I]),
     "sphe_get_lanes": (I, [P, C.POINTER(I)]),

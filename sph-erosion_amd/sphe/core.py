@@ -266,6 +266,15 @@ class Context:
         self._chk(lib().sphe_get_tuning(self._p, C.byref(b), C.byref(r)), "get_tuning")
         return b.value, r.value
 
+    def set_force_block(self, block):
+        """Block size of the force pass (0: the density pass's, as set_tuning leaves it)."""
+        self._chk(lib().sphe_set_force_block(self._p, int(block)), "set_force_block")
+
+    def get_force_block(self):
+        v = C.c_int()
+        self._chk(lib().sphe_get_force_block(self._p, C.byref(v)), "get_force_block")
+        return v.value
+
     def set_lanes(self, lanes):
         self._chk(lib().sphe_set_lanes(self._p, lanes), "set_lanes")
 
@@ -306,6 +315,7 @@ class Context:
         return v.value
 
     def autotune(self, steps_per_trial=5):
+        """Tunes the block sizes; returns (density block, xcd run); get_force_block() has the force pass's."""
         self._chk(lib().sphe_autotune(self._p, steps_per_trial), "autotune")
         return self.get_tuning()
 

@@ -231,21 +231,30 @@ def test_autotune_restores_state_and_counts():
     (regression test of stale counts after a re-upload)."""
     sc = small_dam()
     c1, s1 = gpu_context(sc)
+    c1.profile(True)
+    c1.step(1)
     blk, rm = c1.autotune(2)
+    assert blk in (64, 128, 256, 512) and c1.get_force_block() in (64, 128, 256, 512)
+    # the caller's profile is kept: one step before autotune, none of its trials
+    assert c1.profile_read("density")[1] == 1
+    c1.profile(False)
     c1.step(5)
     c2, s2 = gpu_context(sc)
-    c2.step(5)
+    c2.step(6)
     assert c1.status() == 0 and c2.status() == 0
     p1, v1 = c1.state_f64(s1)
     p2, v2 = c2.state_f64(s2)
     assert np.array_equal(p1, p2) and np.array_equal(v1, v2)
 
 
-@pytest.mark.parametrize("block,remap", [(64, 0), (64, 32), (128, 8), (256, 3), (512, 1)])
-def test_launch_shape_does_not_change_results(block, remap):
+@pytest.mark.parametrize("block,remap,fblock", [(64, 0, 0), (64, 32, 512), (128, 8, 64), (256, 3, 0),
+                                                (512, 1, 128)])
+def test_launch_shape_does_not_change_results(block, remap, fblock):
     sc = small_dam()
     c1, s1 = gpu_context(sc)
     c1.set_tuning(block, remap)
+    c1.set_force_block(fblock)
+    assert c1.get_force_block() == (fblock or block)
     c1.step(4)
     c2, s2 = gpu_context(sc)
     c2.step(4)
