@@ -725,6 +725,14 @@ __device__ __forceinline__ uint32_t group_prefix<4>(uint32_t x, int l, uint32_t*
 }
 
 
+// Dynamic LDS bytes per lane group of the density pass: the row table
+// (W^2 + 1 entries of 8 bytes) and the list staging ring (ring entries of 4
+// bytes); a multiple of 16 (16-byte ring accesses).
+__host__ __device__ constexpr int density_lds_per_group(int W, int ring)
+{
+    return ((W * W + 1) * 8 + ring * 4 + 15) / 16 * 16;
+}
+
 // One G-lane group per particle (lane l = thread index mod G).  The clipped
 // stencil rows form one stream of candidates; lane l takes stream positions
 // l, l+G, ... (U = SPHE_DENSITY_UNROLL per iteration) for its share of the
@@ -770,7 +778,9 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     // so a walk along the table stops by itself.
     const int yz = D.yz, W = 2 * yz + 1, nrows = W * W;
     extern __shared__ uint2 rowtab[];
-    uint2* __restrict__ tab = rowtab + (threadIdx.x / G) * (nrows + 1);
+    constexpr int RING = SPHE_DENSITY_UNROLL * G > 16 ? 64 : 32;  // list staging ring (entries)
+    uint2* __restrict__ tab = (uint2*)((char*)rowtab + (threadIdx.x / G) * density_lds_per_group(W, RING));
+    uint32_t* __restrict__ ring = (uint32_t*)(tab + nrows + 1);
     {
         const RowProbe<T> P = row_probe<T>(D, S, K, pi.x, pi.y, pi.z);
         uint32_t ends = 0;  // stream end of the rounds so far (all lanes)
@@ -818,6 +828,53 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
         const uint32_t total = stream_total;
         int cur = -1;
         uint32_t cur_end = 0, cur_off = 0;  // row cur's stream end and slot offset
+        // Writes list positions c0 .. c0+n-1 (c0 a multiple of 16, n <= 16)
+        // from the ring: lane l copies positions c0 + l PER .. + PER-1, as
+        // 16-byte (G = 8: 8-byte) stores of whole chunks.  The list (capacity
+        // cap, a multiple of 16) continues in the group's pool chunk of cap
+        // entries, claimed when the first chunk past cap is written (one
+        // 64-bit atomic; none left: the rest is not stored and the force
+        // pass walks the stencil); the list rows and pool chunks are 64-byte
+        // aligned.  Called by all lanes of the group.
+        auto flush = [&](uint32_t c0, uint32_t n) {
+            constexpr int PER = 16 / G;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            uint32_t* dst = nullptr;
+            if (c0 < cap) {
+                dst = nbl + c0;
+            } else {
+                if (!claimed) {
+                    claimed = true;
+                    uint32_t b0 = ~0u;
+                    if (l == 0) {
+                        const unsigned long long t = atomicAdd(D.ptop, (unsigned long long)cap);
+                        b0 = t + cap <= (unsigned long long)D.pool_cap ? (uint32_t)t : ~0u;
+                    }
+                    pb = __shfl(b0, gfirst, SPHE_WAVE);
+                }
+                if (pb != ~0u && c0 - cap < cap) dst = D.pool + pb + (c0 - cap);
+            }
+            if (dst) {
+                const uint32_t* src = ring + (c0 & (RING - 1)) + l * PER;
+                uint32_t* out = dst + l * PER;
+                if (n == 16u) {
+                    if constexpr (PER >= 4) {
+#pragma unroll
+                        for (int k = 0; k < PER; k += 4) *(uint4*)(out + k) = *(const uint4*)(src + k);
+                    } else {
+                        *(uint2*)out = *(const uint2*)src;
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < PER; ++k)
+                        if ((uint32_t)(l * PER + k) < n) out[k] = src[k];
+                }
+            }
+            // the ring slots read here are written again only by later blocks
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        };
         auto block = [&](uint32_t base, auto tail) {
             constexpr bool TAIL = decltype(tail)::value;
             uint32_t jj[U];
@@ -870,34 +927,19 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
             const uint32_t excl = group_prefix<G>(x, l, &tot) - x;
             const uint32_t place = excl + (byte_runs(tot) << 8);
             const uint32_t ncnt = __builtin_amdgcn_sad_u8(tot, 0u, cnt);
-            if (ncnt <= cap) {
+            // the entries go to the group's ring in LDS (list position mod
+            // RING); each completed chunk of 16 is written out at once
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if ((x >> (8 * u)) & 1u) nbl[cnt + ((place >> (8 * u)) & 0xFFu)] = jj[u];
-            } else {
-                if (!claimed) {
-                    // first pass over the list capacity: claim maxn pool entries
-                    claimed = true;
-                    uint32_t b0 = ~0u;
-                    if (l == 0) {
-                        const unsigned long long t = atomicAdd(D.ptop, (unsigned long long)cap);
-                        b0 = t + cap <= (unsigned long long)D.pool_cap ? (uint32_t)t : ~0u;
-                    }
-                    pb = __shfl(b0, gfirst, SPHE_WAVE);
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if ((x >> (8 * u)) & 1u) {
-                        const uint32_t m = cnt + ((place >> (8 * u)) & 0xFFu);
-                        if (m < cap) nbl[m] = jj[u];
-                        else if (pb != ~0u && m - cap < cap) D.pool[pb + (m - cap)] = jj[u];
-                    }
-            }
+            for (int u = 0; u < U; ++u)
+                if ((x >> (8 * u)) & 1u) ring[(cnt + ((place >> (8 * u)) & 0xFFu)) & (RING - 1)] = jj[u];
+            // chunks completed by this block: from the one holding position cnt
+            for (uint32_t c0 = cnt & ~15u; c0 + 16u <= ncnt; c0 += 16u) flush(c0, 16u);
             cnt = ncnt;
         };
         uint32_t base = 0;
         for (; base + U * G <= total; base += U * G) block(base, std::false_type());
         if (base < total) block(base, std::true_type());
+        if (cnt & 15u) flush(cnt & ~15u, cnt & 15u);  // the last, partial chunk
     }
     group_merge<G>(sum);
     if (l != 0) return;
@@ -1749,7 +1791,7 @@ static void launch_density_g(const DevState& D, int block, int remap, hipStream_
     // one row table of W^2 + 1 (first slot, length) entries per group, at
     // most 64 KiB per block (halve the block until it fits)
     const int W = 2 * D.yz + 1;
-    const size_t per_group = (size_t)(W * W + 1) * sizeof(uint2);
+    const size_t per_group = (size_t)density_lds_per_group(W, 4 * G > 16 ? 64 : 32);
     while (block > 64 && (size_t)(block / G) * per_group > 65536) block /= 2;
     const size_t lds = (size_t)(block / G) * per_group;
     const int nblk = (int)grid_of(D.P * G, block);
