@@ -725,12 +725,20 @@ __device__ __forceinline__ uint32_t group_prefix<4>(uint32_t x, int l, uint32_t*
 }
 
 
-// Dynamic LDS bytes per lane group of the density pass: the row table
-// (W^2 + 1 entries of 8 bytes) and the list staging ring (ring entries of 4
-// bytes); a multiple of 16 (16-byte ring accesses).
-__host__ __device__ constexpr int density_lds_per_group(int W, int ring)
+// An LDS word addressed by its 32-bit LDS byte address (address space 3).
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+// Dynamic LDS of the density pass for ngroups lane groups: their row tables
+// (W^2 + 1 entries of 8 bytes each), then their list staging rings (ring
+// entries of 4 bytes each) from an offset aligned to a ring's size, so a
+// ring slot's address is the ring's base or-ed with the slot's byte offset.
+__host__ __device__ constexpr int density_ring_offset(int W, int ngroups, int ring)
 {
-    return ((W * W + 1) * 8 + ring * 4 + 15) / 16 * 16;
+    return (ngroups * (W * W + 1) * 8 + ring * 4 - 1) / (ring * 4) * (ring * 4);
+}
+__host__ __device__ constexpr int density_lds_bytes(int W, int ngroups, int ring)
+{
+    return density_ring_offset(W, ngroups, ring) + ngroups * ring * 4;
 }
 
 // One G-lane group per particle (lane l = thread index mod G).  The clipped
@@ -777,10 +785,16 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     // the stream ends.  Entry W^2 is a sentinel row that ends at 2^32 - 1,
     // so a walk along the table stops by itself.
     const int yz = D.yz, W = 2 * yz + 1, nrows = W * W;
-    extern __shared__ uint2 rowtab[];
+    extern __shared__ __attribute__((aligned(256))) uint2 rowtab[];
     constexpr int RING = SPHE_DENSITY_UNROLL * G > 16 ? 64 : 32;  // list staging ring (entries)
-    uint2* __restrict__ tab = (uint2*)((char*)rowtab + (threadIdx.x / G) * density_lds_per_group(W, RING));
-    uint32_t* __restrict__ ring = (uint32_t*)(tab + nrows + 1);
+    constexpr uint32_t RMASK = RING * 4 - 1;                       // byte offset in a ring
+    const int grp = (int)(threadIdx.x / G);
+    uint2* __restrict__ tab = rowtab + grp * (nrows + 1);
+    // the group's ring, size-aligned: slot offset ring_rel | (4 * position
+    // mod RING) in the array; the hot stores use the LDS byte address
+    // ring_abs | ..., the same with the array's own (256-byte aligned) address
+    const uint32_t ring_rel = (uint32_t)(density_ring_offset(W, (int)(blockDim.x / G), RING) + grp * RING * 4);
+    const uint32_t ring_abs = (uint32_t)(uintptr_t)rowtab + ring_rel;
     {
         const RowProbe<T> P = row_probe<T>(D, S, K, pi.x, pi.y, pi.z);
         uint32_t ends = 0;  // stream end of the rounds so far (all lanes)
@@ -857,7 +871,7 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
                 if (pb != ~0u && c0 - cap < cap) dst = D.pool + pb + (c0 - cap);
             }
             if (dst) {
-                const uint32_t* src = ring + (c0 & (RING - 1)) + l * PER;
+                const uint32_t* src = (const uint32_t*)((char*)rowtab + (ring_rel | ((c0 * 4u) & RMASK))) + l * PER;
                 uint32_t* out = dst + l * PER;
                 if (n == 16u) {
                     if constexpr (PER >= 4) {
@@ -928,12 +942,22 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
             const uint32_t place = excl + (byte_runs(tot) << 8);
             const uint32_t ncnt = __builtin_amdgcn_sad_u8(tot, 0u, cnt);
             // the entries go to the group's ring in LDS (list position mod
-            // RING); each completed chunk of 16 is written out at once
+            // RING, as byte offsets: place's bytes stay below 64, so times 4
+            // they do not carry); each completed chunk of 16 is written out
+            // at once
+            const uint32_t place4 = place << 2, cnt4 = cnt << 2;
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                if ((x >> (8 * u)) & 1u) ring[(cnt + ((place >> (8 * u)) & 0xFFu)) & (RING - 1)] = jj[u];
-            // chunks completed by this block: from the one holding position cnt
-            for (uint32_t c0 = cnt & ~15u; c0 + 16u <= ncnt; c0 += 16u) flush(c0, 16u);
+                if ((x >> (8 * u)) & 1u)
+                    *(lds_u32*)(uintptr_t)(ring_abs | ((cnt4 + ((place4 >> (8 * u)) & 0xFFu)) & RMASK)) = jj[u];
+            // chunks completed by this block: from the one holding position
+            // cnt (one at most when a block holds 16 positions or fewer)
+            if constexpr (U * G <= 16) {
+                const uint32_t c0 = cnt & ~15u;
+                if (c0 + 16u <= ncnt) flush(c0, 16u);
+            } else {
+                for (uint32_t c0 = cnt & ~15u; c0 + 16u <= ncnt; c0 += 16u) flush(c0, 16u);
+            }
             cnt = ncnt;
         };
         uint32_t base = 0;
@@ -1788,12 +1812,11 @@ void sphe_launch_reorder(const DevState& D, hipStream_t st, hipEvent_t e0, hipEv
 template <int G>
 static void launch_density_g(const DevState& D, int block, int remap, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
-    // one row table of W^2 + 1 (first slot, length) entries per group, at
-    // most 64 KiB per block (halve the block until it fits)
-    const int W = 2 * D.yz + 1;
-    const size_t per_group = (size_t)density_lds_per_group(W, 4 * G > 16 ? 64 : 32);
-    while (block > 64 && (size_t)(block / G) * per_group > 65536) block /= 2;
-    const size_t lds = (size_t)(block / G) * per_group;
+    // one row table and one list staging ring per group, at most 64 KiB per
+    // block (halve the block until it fits)
+    const int W = 2 * D.yz + 1, ring = 4 * G > 16 ? 64 : 32;
+    while (block > 64 && density_lds_bytes(W, block / G, ring) > 65536) block /= 2;
+    const size_t lds = (size_t)density_lds_bytes(W, block / G, ring);
     const int nblk = (int)grid_of(D.P * G, block);
     SPHE_PREC_SWITCH(D.precision,
                      (klaunch_lds(k_density<float, true, G>, nblk, block, lds, st, e0, e1, D, nblk, remap)),
