@@ -840,8 +840,9 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     {
         constexpr int U = SPHE_DENSITY_UNROLL;
         const uint32_t total = stream_total;
-        int cur = -1;
-        uint32_t cur_end = 0, cur_off = 0;  // row cur's stream end and slot offset
+        const uint2* cur = tab - 1;         // the current row's entry (none yet)
+        uint32_t cur_end = 0, cur_off = 0;  // its stream end and slot offset
+        const uint32_t lane_b = (uint32_t)l * VB;
         // Writes list positions c0 .. c0+n-1 (c0 a multiple of 16, n <= 16)
         // from the ring: lane l copies positions c0 + l PER .. + PER-1, as
         // 16-byte (G = 8: 8-byte) stores of whole chunks.  The list (capacity
@@ -896,9 +897,9 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t t = base + (uint32_t)(u * G + l);
-                const uint32_t tb = t * VB;
+                const uint32_t tb = base * VB + (uint32_t)(u * G) * VB + lane_b;
                 while (tb >= cur_end) {
-                    const uint2 e = tab[++cur];
+                    const uint2 e = *++cur;
                     cur_end = e.x;
                     cur_off = e.y;
                 }
