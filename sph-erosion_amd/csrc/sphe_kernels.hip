@@ -747,7 +747,9 @@ __host__ __device__ constexpr int density_lds_bytes(int W, int ngroups, int ring
 // density sum.
 // The group writes one neighbour list per particle (self left out) in
 // canonical order, nbl[slot * maxn + m], m < maxn, count in nbc[slot]; a
-// byte-wise prefix sum over the group places each lane's entries.  List
+// byte-wise prefix sum over the group places each lane's entries in a ring
+// in LDS, and every completed chunk of 16 entries goes out with one 16-byte
+// store per lane (64-byte aligned: one cache line per group).  List
 // entries are byte offsets of the neighbours' slots in the per-slot vectors
 // of V4<T> (slot * sizeof(V4<T>), below 2^32 by the layout's limits).  The
 // group then merges the partial sums and lane 0 writes rho and
@@ -780,7 +782,8 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     // total-1 in canonical order.  Lane l computes rows l, l+G, ... (the
     // group takes G rows per round) and writes (stream end, slot offset),
     // both in bytes (times VB), to the group's row table in LDS (dynamic:
-    // blockDim / G tables of W^2 + 1 entries): positions t of row r lie below
+    // blockDim / G tables of W^2 + 1 entries, then the groups' rings, laid
+    // out by density_ring_offset): positions t of row r lie below
     // its end and hold slot t + offset.  A group prefix sum per round gives
     // the stream ends.  Entry W^2 is a sentinel row that ends at 2^32 - 1,
     // so a walk along the table stops by itself.
