@@ -179,6 +179,20 @@ template <typename T> struct Acc<T, false> {
     __device__ __forceinline__ T value() const { return s; }
 };
 
+// FP64 sum of FP32 terms (plain adds; the lanes of a pair merge with the
+// same commutative add, so both hold identical bits), rounded to FP32 once.
+struct AccF64 {
+    double s = 0.0;
+    __device__ __forceinline__ void add(float v) { s = s + (double)v; }
+    template <int M>
+    __device__ __forceinline__ void merge_xor()
+    {
+        const double os = lane_xor<M>(s);
+        s = s + os;
+    }
+    __device__ __forceinline__ float value() const { return (float)s; }
+};
+
 template <int G, typename A>
 __device__ __forceinline__ void group_merge(A& a)
 {
@@ -1066,12 +1080,18 @@ __device__ __forceinline__ Terms<T> pair_terms(const SceneConsts<T>& K, T dx, T 
 
 // Force sums of one lane.  A lane's neighbours enter in consecutive pairs:
 // the two terms are added in plain arithmetic (first + second) and the pair
-// sum enters the compensated sum; an odd last neighbour enters alone.
+// sum enters the lane's sum; an odd last neighbour enters alone.
 // finish() applies the constants and forms the acceleration
 // (fluid_system.h:151-154, 163, 177, 318-321).
+// The force sums' accumulator: FP32 compensated mode sums in FP64 (two
+// instructions per term instead of the four of a compensated add, and more
+// precise), the other modes as their density sums.
+template <typename T, bool COMP> struct ForceAccSel { typedef Acc<T, COMP> type; };
+template <> struct ForceAccSel<float, true> { typedef AccF64 type; };
+
 template <typename T, bool COMP>
 struct ForceAcc {
-    Acc<T, COMP> fp0, fp1, fp2, fv0, fv1, fv2, nn0, nn1, nn2, lap;
+    typename ForceAccSel<T, COMP>::type fp0, fp1, fp2, fv0, fv1, fv2, nn0, nn1, nn2, lap;
 
     // The particle's own colour-Laplacian term (d = 0): added first, by lane 0.
     __device__ __forceinline__ void self_term(const SceneConsts<T>& K, T b_i)
