@@ -276,7 +276,7 @@ def main():
                ms_per_step=1e3 * elapsed / a.steps, higher_is_better=True, scaling="weak",
                vs_baseline=None, dtype="fp32", data="synthetic",
                config=dict(workload=f"{a.workload}: {desc}", particles=N, scenes=len(scs),
-                           precision="fp32 compensated (Kahan)", block=blk, force_block=fblk, density_xcd_run=rm,
+                           precision="fp32 (compensated density sums, FP64-accumulated force sums)", block=blk, force_block=fblk, density_xcd_run=rm,
                            x_slabs=ctx.get_subcells(), yz_split=ctx.get_yz_split()),
                roofline=roofline)
     out["value_kernel_by_kernel"] = N * a.steps / prof_elapsed

@@ -31,7 +31,7 @@ extern "C" {
 #define SPHE_ERANGE (-5)  /* index / id out of range */
 
 /* Arithmetic of the step. */
-#define SPHE_FP32 0        /* FP32, Kahan-compensated neighbour sums */
+#define SPHE_FP32 0        /* FP32; compensated density sums, FP64-accumulated force sums */
 #define SPHE_FP32_NAIVE 1  /* FP32, plain sums */
 #define SPHE_FP64 2        /* plain FP64 comparison path */
 
