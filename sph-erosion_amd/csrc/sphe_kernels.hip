@@ -1297,8 +1297,15 @@ __device__ __forceinline__ void force_list(const DevState& D, const SceneConsts<
     const V4<T>* __restrict__ posa = (const V4<T>*)D.posa;
     const V4<T>* __restrict__ vel = (const V4<T>*)D.s_vel;
     uint32_t k = (uint32_t)l;
+    // the list entries of the next four-neighbour iteration load while the
+    // current one computes
+    uint32_t na = 0, nb = 0, nc = 0, nd = 0;
+    if (k + 3 * G < cnt) { na = fetch(k); nb = fetch(k + G); nc = fetch(k + 2 * G); nd = fetch(k + 3 * G); }
     for (; k + 3 * G < cnt; k += 4 * G) {
-        const uint32_t ja = fetch(k), jb = fetch(k + G), jc = fetch(k + 2 * G), jd = fetch(k + 3 * G);
+        const uint32_t ja = na, jb = nb, jc = nc, jd = nd;
+        if (k + 7 * G < cnt) {
+            na = fetch(k + 4 * G); nb = fetch(k + 5 * G); nc = fetch(k + 6 * G); nd = fetch(k + 7 * G);
+        }
         const V4<T> pa = at_byte(posa, ja), va = at_byte(vel, ja), pb = at_byte(posa, jb), vb = at_byte(vel, jb);
         const V4<T> pc = at_byte(posa, jc), vc = at_byte(vel, jc), pd = at_byte(posa, jd), vd = at_byte(vel, jd);
         {
