@@ -3,8 +3,10 @@
 // Same method names and signatures as Erosion/fluid_system.h:66-485, backed
 // by the C ABI (sphe.h) and the MI355X kernels.  A front end written against
 // the reference (Erosion/main.cpp) compiles against this header after
-// replacing #include "fluid_system.h"; see INTEGRATION.md for the two calls
-// that change (Draw, which is GL rendering, and the optional terrain).
+// replacing #include "fluid_system.h".  Two calls change: Draw (GL
+// rendering, which stays with the front end: it draws the positions that
+// GetVertices returns) and the optional terrain, which the reference does
+// not couple to the fluid.
 //
 // Errors: a failing call throws std::runtime_error with sphe_last_error().
 // When glm is included first (GLM_VERSION defined), GetGrav / GetOrigin /
