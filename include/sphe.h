@@ -31,9 +31,10 @@ extern "C" {
 #define SPHE_ERANGE (-5)  /* index / id out of range */
 
 /* Arithmetic of the step. */
-#define SPHE_FP32 0        /* FP32; compensated density sums, FP64-accumulated force sums */
-#define SPHE_FP32_NAIVE 1  /* FP32, plain sums */
-#define SPHE_FP64 2        /* plain FP64 comparison path */
+#define SPHE_FP32 0         /* FP32; Kahan-compensated density and force sums */
+#define SPHE_FP32_NAIVE 1   /* FP32, plain sums */
+#define SPHE_FP64 2         /* plain FP64 comparison path */
+#define SPHE_FP32_F64ACC 3  /* FP32 terms; compensated density sums, force sums accumulated in FP64 */
 
 /* Erosion pass. */
 #define SPHE_EROSION_OFF 0
@@ -109,7 +110,8 @@ typedef struct sphe_ctx sphe_ctx;
 /* ---- lifetime ---------------------------------------------------------- */
 
 /* FluidSystemSPH::FluidSystemSPH (fluid_system.h:69-72).  precision is one of
- * SPHE_FP32, SPHE_FP32_NAIVE, SPHE_FP64.  Uses the current HIP device. */
+ * SPHE_FP32, SPHE_FP32_NAIVE, SPHE_FP64, SPHE_FP32_F64ACC.  Uses the current
+ * HIP device. */
 int sphe_create(int precision, sphe_ctx** out);
 int sphe_destroy(sphe_ctx* ctx);
 const char* sphe_last_error(const sphe_ctx* ctx);
@@ -169,8 +171,15 @@ int sphe_read_state_f64(sphe_ctx* ctx, int scene, double* pos3, double* vel3);
 /* Fixed-point sediment (1 unit = 2^-sphe_fixed_exponent volume), id order. */
 int sphe_read_sediment_fx(sphe_ctx* ctx, int scene, int64_t* sed_fx);
 /* FluidSystemSPH::GetParticle (fluid_system.h:286-289): the 112-byte record
- * of particle `id` after the last step (forces of that step).  NeighbId is the
- * largest neighbour id, -1 when the particle had no neighbour. */
+ * of particle `id` after the last step (forces of that step, summed in the
+ * order the step summed them).  NeighbId is the largest neighbour id, -1 when
+ * the particle had no neighbour.  Costs one small kernel and one 232-byte
+ * read-back; the first call after a step also rebuilds the context's
+ * id -> slot map on the device (one pass over the slots).  Ids that are
+ * sparse (largest id above 2 n + 4096) are found by a search over the
+ * scene's slots instead.  With duplicate ids (Initialize on a populated
+ * scene) the record is one of them.  Before any step (or after an edit) the
+ * record has the position and velocity and zero forces. */
 int sphe_get_particle(sphe_ctx* ctx, int scene, int id, sphe_particle* out);
 
 /* ---- terrain ----------------------------------------------------------- */
@@ -180,16 +189,20 @@ int sphe_get_particle(sphe_ctx* ctx, int scene, int id, sphe_particle* out);
  * front end holds its heightfield in Grid (grid.h:26-51) and never couples it
  * to the fluid; this is the coupling. */
 int sphe_set_terrain(sphe_ctx* ctx, int scene, const sphe_terrain_desc* terrain);
-/* Grid::LoadHeightfield (grid.h:98-102): uint8 heights of a w x h image,
- * map[w_index * h + h_index] as Grid::GetHeightfieldAt reads it
- * (grid.h:104-107), scaled by `scale` and offset by the terrain origin y. */
+/* Grid::LoadHeightfield (grid.h:98-102): uint8 heights of a row-major
+ * image of w pixels per row and h rows (stbi_load's layout, main.cpp:88-92).
+ * Terrain cell (x, z) takes Grid::GetHeightfieldAt(x, z) = img[w * x + z]
+ * (grid.h:104-107: x picks the row, z the column), so the terrain has h cells
+ * along x and w along z; height = origin y + scale * pixel.  The terrain's
+ * origin and cell edge are kept (cell 1 if none was set). */
 int sphe_terrain_load_u8(sphe_ctx* ctx, int scene, const uint8_t* img, int w, int h, float scale);
 /* Heights (absolute y) of the scene's cells, row = z. */
 int sphe_terrain_heights(sphe_ctx* ctx, int scene, float* out, size_t count);
 /* Fixed-point cell volumes, row = z. */
 int sphe_terrain_volume_fx(sphe_ctx* ctx, int scene, int64_t* out, size_t count);
 /* Restores fixed-point cell volumes read by sphe_terrain_volume_fx (count =
- * nx * nz, 0 <= v <= 2^56, the scene's fixed exponent unchanged): with
+ * nx * nz, |v| <= 2^56, the scene's fixed exponent unchanged; erosion is
+ * not capped at a cell's volume, so a cell may be below the terrain base): with
  * sphe_set_particles_f64 (sediment included) a checkpoint of a scene, from
  * which the steps continue bit for bit.  No counterpart in the reference,
  * whose terrain is static. */

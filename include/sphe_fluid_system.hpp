@@ -131,7 +131,10 @@ public:
 
     // Terrain coupling (heightfield collision + erosion): Grid::LoadHeightfield
     // semantics (grid.h:98-107) with a height scale, then the eroded surface
-    // in Grid::UpdateGrid / GetSurfaceParts / GetIndices form.
+    // in Grid::UpdateGrid / GetSurfaceParts / GetIndices form.  img, w, h as
+    // stbi_load returns them (row-major, w pixels per row, h rows); cell
+    // (x, z) is GetHeightfieldAt(x, z) = img[w * x + z]: h cells along x, w
+    // along z.
     void LoadHeightfield(const unsigned char* img, int w, int h, float scale, float cell, const float origin[3],
                          int erosion = SPHE_EROSION_ATOMIC)
     {
@@ -140,8 +143,8 @@ public:
         p.erosion = erosion;
         check(sphe_set_params(ctx_, scene_, &p), "set_params");
         sphe_terrain_desc t{};
-        t.nx = w;
-        t.nz = h;
+        t.nx = h;
+        t.nz = w;
         for (int a = 0; a < 3; ++a) t.origin[a] = origin[a];
         t.cell = cell;
         t.heights = nullptr;

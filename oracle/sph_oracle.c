@@ -414,7 +414,7 @@ void orc_derive(const orc_scene* s, orc_consts* c)
         c->band = (float)s->band;
         c->up = (float)ldexp(1.0, s->fx_exp);
         c->dn = (float)ldexp(1.0, -s->fx_exp);
-        c->dmax = (float)ldexp(1.0, 44 - s->fx_exp);
+        c->dmax = (float)ldexp(1.0, 52 - s->fx_exp);
     }
 }
 

@@ -163,7 +163,8 @@ typedef struct {
 
 /* One step of the build's algorithm for one scene, in place, input order
  * preserved.  real = 8 runs the FP64 oracle, real = 4 the FP32 twin.
- * comp = 1 uses Kahan-compensated neighbour sums (FP32 twin only).
+ * comp (FP32 twin only): 0 plain sums, 1 Kahan-compensated density and force
+ * sums, 2 compensated density sums and FP64-accumulated force sums.
  * pos/vel hold 3 doubles per particle (FP32 values when real == 4).
  * vol_fx (tnx*tnz, row = z) and height (float, same shape) are the terrain. */
 int orc_step(const orc_scene* s, int real, int comp, int n,

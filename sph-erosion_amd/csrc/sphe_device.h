@@ -39,7 +39,10 @@ struct SceneDev {
     int32_t t_off, tnx, tnz, erosion, fx_exp;
     float hq;    // height = t_org[1] + (float)vol * hq
     float t_y0;
-    int32_t pad_[2];
+    // id -> sorted slot map of the last step (sphe_get_particle): entries
+    // [idmap_off, idmap_off + idmap_len) for ids 0 .. idmap_len - 1; -1: the
+    // scene's ids are too sparse for a map and are searched instead
+    int32_t idmap_off, idmap_len;
     SceneConsts<float> f;
     SceneConsts<double> d;
 };

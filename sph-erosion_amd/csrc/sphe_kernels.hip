@@ -179,6 +179,13 @@ template <typename T> struct Acc<T, false> {
     __device__ __forceinline__ T value() const { return s; }
 };
 
+// Summation modes of the neighbour sums (the precision argument of
+// sphe_create picks one): plain; Kahan-compensated density and force sums;
+// compensated density sums with the force sums accumulated in FP64.
+#define SUM_PLAIN 0
+#define SUM_KAHAN 1
+#define SUM_F64ACC 2
+
 // FP64 sum of FP32 terms (plain adds; the lanes of a pair merge with the
 // same commutative add, so both hold identical bits), rounded to FP32 once.
 struct AccF64 {
@@ -768,7 +775,7 @@ __host__ __device__ constexpr int density_lds_bytes(int W, int ngroups, int ring
 // of V4<T> (slot * sizeof(V4<T>), below 2^32 by the layout's limits).  The
 // group then merges the partial sums and lane 0 writes rho and
 // (p / rho^2, m / rho).
-template <typename T, bool COMP, int G>
+template <typename T, int SM, int G>
 __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap)
 {
     const int b = xcd_block(blockIdx.x, nblk, remap);
@@ -781,7 +788,7 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     const SceneConsts<T>& K = SceneSel<T>::get(S);
     const V4<T>* __restrict__ pos = (const V4<T>*)D.s_pos;
     const V4<T> pi = pos[slot];
-    Acc<T, COMP> sum;
+    Acc<T, SM != SUM_PLAIN> sum;
     const T h2 = K.h2, thr = K.thr;
     const uint32_t cap = (uint32_t)D.maxn;
     uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn;
@@ -1083,15 +1090,15 @@ __device__ __forceinline__ Terms<T> pair_terms(const SceneConsts<T>& K, T dx, T 
 // sum enters the lane's sum; an odd last neighbour enters alone.
 // finish() applies the constants and forms the acceleration
 // (fluid_system.h:151-154, 163, 177, 318-321).
-// The force sums' accumulator: FP32 compensated mode sums in FP64 (two
-// instructions per term instead of the four of a compensated add, and more
-// precise), the other modes as their density sums.
-template <typename T, bool COMP> struct ForceAccSel { typedef Acc<T, COMP> type; };
-template <> struct ForceAccSel<float, true> { typedef AccF64 type; };
+// The force sums' accumulator: plain, Kahan-compensated, or (FP32 terms)
+// FP64 sums (two instructions per term instead of the four of a
+// compensated add) rounded to FP32 once.
+template <typename T, int SM> struct ForceAccSel { typedef Acc<T, SM != SUM_PLAIN> type; };
+template <> struct ForceAccSel<float, SUM_F64ACC> { typedef AccF64 type; };
 
-template <typename T, bool COMP>
+template <typename T, int SM>
 struct ForceAcc {
-    typename ForceAccSel<T, COMP>::type fp0, fp1, fp2, fv0, fv1, fv2, nn0, nn1, nn2, lap;
+    typename ForceAccSel<T, SM>::type fp0, fp1, fp2, fv0, fv1, fv2, nn0, nn1, nn2, lap;
 
     // The particle's own colour-Laplacian term (d = 0): added first, by lane 0.
     __device__ __forceinline__ void self_term(const SceneConsts<T>& K, T b_i)
@@ -1147,10 +1154,10 @@ struct ForceAcc {
 // visits all candidates in canonical order and takes the accepted
 // neighbours m = l, l+G, l+2G, ... (self excluded), as it would from the
 // list.  count and nbmax cover the lane's share (self counted by lane 0).
-template <typename T, bool COMP, int G>
+template <typename T, int SM, int G>
 __device__ __forceinline__ void force_share_walk(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
                                                  int64_t slot, int l, const V4<T>& pi, const V4<T>& vi, T a_i,
-                                                 ForceAcc<T, COMP>& acc, int32_t& nbmax, int32_t& count)
+                                                 ForceAcc<T, SM>& acc, int32_t& nbmax, int32_t& count)
 {
     const V4<T>* __restrict__ pos = (const V4<T>*)D.s_pos;
     const V4<T>* __restrict__ posa = (const V4<T>*)D.posa;
@@ -1278,7 +1285,8 @@ __device__ __forceinline__ int64_t erode_exchange(const SceneDev& S, const Scene
     T d;
     if (cap > s) d = K.ke * (cap - s);
     else d = -(K.kd * (s - cap));
-    if (!(d_fabs(d) <= K.dmax)) d = T(0);
+    // one exchange saturates at 2^52 units (NaN: none), far inside int64
+    if (!(d_fabs(d) <= K.dmax)) d = d > T(0) ? K.dmax : (d < T(0) ? -K.dmax : T(0));
     int64_t dq = (int64_t)d_rint(d * K.up);
     if (dq < -sed) dq = -sed;
     return dq;
@@ -1289,9 +1297,9 @@ __device__ __forceinline__ int64_t erode_exchange(const SceneDev& S, const Scene
 // neighbour's slot in the V4<T> vectors): the lanes
 // of a particle differ by one neighbour at most.  Two pairs per iteration
 // (eight gathers in flight), then one pair, then a single.
-template <typename T, bool COMP, int G, typename F>
+template <typename T, int SM, int G, typename F>
 __device__ __forceinline__ void force_list(const DevState& D, const SceneConsts<T>& K, int l, uint32_t cnt,
-                                          const V4<T>& pi, const V4<T>& vi, T a_i, ForceAcc<T, COMP>& acc,
+                                          const V4<T>& pi, const V4<T>& vi, T a_i, ForceAcc<T, SM>& acc,
                                           F&& fetch)
 {
     const V4<T>* __restrict__ posa = (const V4<T>*)D.posa;
@@ -1352,33 +1360,33 @@ __device__ __forceinline__ void force_list(const DevState& D, const SceneConsts<
 // Lane l's share of the force sums of one particle: from its list, the
 // list continued in its overflow chunk, or (no room for all neighbours)
 // by walking the stencil.
-template <typename T, bool COMP, int G>
+template <typename T, int SM, int G>
 __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
                                            int64_t slot, int l, const V4<T>& pi, const V4<T>& vi, T a_i,
-                                           ForceAcc<T, COMP>& acc)
+                                           ForceAcc<T, SM>& acc)
 {
     const uint32_t cnt = D.nbc[slot];
     const uint32_t cap = (uint32_t)D.maxn;
     const uint32_t* __restrict__ nbl = D.nbl + slot * (int64_t)D.maxn;
     if (cnt <= cap) {
-        force_list<T, COMP, G>(D, K, l, cnt, pi, vi, a_i, acc, [&](uint32_t k) { return nbl[k]; });
+        force_list<T, SM, G>(D, K, l, cnt, pi, vi, a_i, acc, [&](uint32_t k) { return nbl[k]; });
         return;
     }
     const uint32_t pb = D.pbase[slot];
     if (pb != ~0u && cnt - cap <= cap) {
         const uint32_t* __restrict__ pool = D.pool;
-        force_list<T, COMP, G>(D, K, l, cnt, pi, vi, a_i, acc,
+        force_list<T, SM, G>(D, K, l, cnt, pi, vi, a_i, acc,
                                [&](uint32_t k) { return k < cap ? nbl[k] : pool[pb + (k - cap)]; });
         return;
     }
     int32_t nbmax = -1, count = 0;
-    force_share_walk<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc, nbmax, count);
+    force_share_walk<T, SM, G>(D, S, K, slot, l, pi, vi, a_i, acc, nbmax, count);
 }
 
 // One G-lane group per particle: the lanes sum their shares, merge them
 // (xor shuffles), and lane 0 integrates, collides, erodes, stores the new
 // state and counts the particle's next cell.
-template <typename T, bool COMP, int G>
+template <typename T, int SM, int G>
 __device__ __forceinline__ void force_body(const DevState& D, int nblk, int remap)
 {
     const int b = xcd_block(blockIdx.x, nblk, remap);
@@ -1407,9 +1415,9 @@ __device__ __forceinline__ void force_body(const DevState& D, int nblk, int rema
     const V4<T> vi = svel[slot];
     const T rho_i = ((const T*)D.rho)[slot];
     const T a_i = ((const V4<T>*)D.posa)[slot].w;
-    ForceAcc<T, COMP> acc;
+    ForceAcc<T, SM> acc;
     if (l == 0) acc.self_term(K, vi.w);
-    force_share<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc);
+    force_share<T, SM, G>(D, S, K, slot, l, pi, vi, a_i, acc);
     acc.template merge<G>();
     if (l != 0) return;
     ForceOut<T> o;
@@ -1462,17 +1470,17 @@ __device__ __forceinline__ void force_body(const DevState& D, int nblk, int rema
 // only the rarely taken stencil walk of overflowing particles spills.
 // Unbounded, the walk raises the count and the occupancy of the whole
 // kernel drops to 4.
-template <bool COMP, int G>
+template <int SM, int G>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_force_f32(DevState D, int nblk,
                                                                                               int remap)
 {
-    force_body<float, COMP, G>(D, nblk, remap);
+    force_body<float, SM, G>(D, nblk, remap);
 }
 
 template <int G>
 __global__ __launch_bounds__(512) void k_force_f64(DevState D, int nblk, int remap)
 {
-    force_body<double, false, G>(D, nblk, remap);
+    force_body<double, SUM_PLAIN, G>(D, nblk, remap);
 }
 
 // --------------------------------------------------------------- terrain
@@ -1558,11 +1566,19 @@ __global__ __launch_bounds__(256) void k_tapply(DevState D)
 
 // The record of one particle after a step: its group recomputes the force
 // sums of that step (same lane split and merge as k_force) from the retained
-// sorted state.
-template <typename T, bool COMP, int G>
-__global__ void k_inspect(DevState D, int64_t slot, double* out)
+// sorted state, and reads the state the step left in the same slot.  The
+// slot comes from device memory (the id map or the id search; INT32_MAX or
+// out of range: no such particle, out[22] = -1).
+template <typename T, int SM, int G>
+__global__ void k_inspect(DevState D, const int32_t* __restrict__ slotp, double* out)
 {
     if (blockIdx.x != 0 || threadIdx.x >= G) return;
+    const int32_t s32 = *slotp;
+    if (s32 < 0 || (int64_t)s32 >= D.P) {  // the same in every lane
+        if (threadIdx.x == 0) out[22] = -1.0;
+        return;
+    }
+    const int64_t slot = s32;
     const int l = (int)threadIdx.x;
     const SceneDev& S = D.scenes[D.chunk_scene[slot >> 6]];
     const SceneConsts<T>& K = SceneSel<T>::get(S);
@@ -1570,10 +1586,10 @@ __global__ void k_inspect(DevState D, int64_t slot, double* out)
     const V4<T> vi = ((const V4<T>*)D.s_vel)[slot];
     const T rho_i = ((const T*)D.rho)[slot];
     const T a_i = ((const V4<T>*)D.posa)[slot].w;
-    ForceAcc<T, COMP> acc;
+    ForceAcc<T, SM> acc;
     if (l == 0) acc.self_term(K, vi.w);
     int32_t nbmax = -1, count = 0;
-    force_share_walk<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc, nbmax, count);
+    force_share_walk<T, SM, G>(D, S, K, slot, l, pi, vi, a_i, acc, nbmax, count);
     acc.template merge<G>();
 #pragma unroll
     for (int m = 1; m < G; m <<= 1) {
@@ -1597,13 +1613,48 @@ __global__ void k_inspect(DevState D, int64_t slot, double* out)
     }
     out[20] = (double)o.nbmax;
     out[21] = (double)o.count;
+    out[22] = (double)slot;
+    const V4<T> np = ((const V4<T>*)D.u_pos)[slot];
+    const V4<T> nv = ((const V4<T>*)D.u_vel)[slot];
+    out[23] = (double)np.x; out[24] = (double)np.y; out[25] = (double)np.z;
+    out[26] = (double)nv.x; out[27] = (double)nv.y; out[28] = (double)nv.z;
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// id -> sorted slot of the last step, for the scenes with a map (cleared to
+// INT32_MAX first; duplicate ids keep their lowest slot).
+__global__ __launch_bounds__(256) void k_idmap(DevState D, int32_t* __restrict__ map)
+{
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= D.P) return;
+    const SceneDev& S = D.scenes[wave_scene(D.chunk_scene, slot)];
+    if (slot >= S.p_begin + S.n || S.idmap_off < 0) return;
+    const int32_t id = D.s_id[slot];
+    if (id >= 0 && id < S.idmap_len) atomicMin(&map[S.idmap_off + id], (int32_t)slot);
+}
+
+// The sorted slot of particle `id` of one scene without a map (lowest slot
+// for a duplicate id), into *slot_out (cleared to INT32_MAX first).
+__global__ __launch_bounds__(256) void k_find_id(DevState D, int scene, int32_t id, int32_t* __restrict__ slot_out)
+{
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= D.P) return;
+    const int si = wave_scene(D.chunk_scene, slot);
+    if (si != scene) return;
+    const SceneDev& S = D.scenes[si];
+    if (slot < S.p_begin + S.n && D.s_id[slot] == id) atomicMin(slot_out, (int32_t)slot);
 }
 
 // Acceleration of every particle of one scene in the current sorted state
 // (density pass done): the force pass's sums (list, pool or walk, the same
 // order and lane split) and finish(), without integration.
 // out[3 (slot - p_begin) + a], sorted order.
-template <typename T, bool COMP, int G>
+template <typename T, int SM, int G>
 __global__ __launch_bounds__(256) void k_accel(DevState D, int scene, double* __restrict__ out)
 {
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1619,9 +1670,9 @@ __global__ __launch_bounds__(256) void k_accel(DevState D, int scene, double* __
     const V4<T> vi = ((const V4<T>*)D.s_vel)[slot];
     const T rho_i = ((const T*)D.rho)[slot];
     const T a_i = ((const V4<T>*)D.posa)[slot].w;
-    ForceAcc<T, COMP> acc;
+    ForceAcc<T, SM> acc;
     if (l == 0) acc.self_term(K, vi.w);
-    force_share<T, COMP, G>(D, S, K, slot, l, pi, vi, a_i, acc);
+    force_share<T, SM, G>(D, S, K, slot, l, pi, vi, a_i, acc);
     acc.template merge<G>();
     if (l != 0) return;
     ForceOut<T> o;
@@ -1716,6 +1767,14 @@ __global__ void k_clear_status(uint32_t* status)
     if (threadIdx.x == 0 && blockIdx.x == 0) *status = 0u;
 }
 
+// Copies the list-overflow word into host memory mapped for the device (a
+// vector store), where the host reads it between step sequences without
+// waiting for the stream.
+__global__ void k_publish(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *(volatile uint32_t*)dst = *src;
+}
+
 // Mass bookkeeping of one scene: FP64 sums of terrain volume and carried
 // sediment, and the exact mod-2^64 sum of their fixed-point units.  Each
 // block writes its partial; k_mass_final adds them in block order.
@@ -1780,11 +1839,15 @@ inline unsigned grid_of(int64_t n, int block) { return (unsigned)((n + block - 1
 
 // ------------------------------------------------------------- launchers
 
-#define SPHE_PREC_SWITCH(prec, CALL_F32C, CALL_F32N, CALL_F64) \
-    do {                                                        \
-        if ((prec) == 0) { CALL_F32C; }                         \
-        else if ((prec) == 1) { CALL_F32N; }                    \
-        else { CALL_F64; }                                      \
+// The precision modes of sphe_create: 0 FP32 with Kahan-compensated sums,
+// 1 FP32 plain, 2 FP64, 3 FP32 with compensated density sums and
+// FP64-accumulated force sums.  KERN(T, SM) names the kernel instantiation.
+#define SPHE_PREC_SWITCH(prec, CALL)                                      \
+    do {                                                                  \
+        if ((prec) == 0) { CALL(float, SUM_KAHAN); }                      \
+        else if ((prec) == 1) { CALL(float, SUM_PLAIN); }                 \
+        else if ((prec) == 3) { CALL(float, SUM_F64ACC); }                \
+        else { CALL(double, SUM_PLAIN); }                                 \
     } while (0)
 
 // Launches kern.  With profiling events, through hipExtLaunchKernel: e0 and
@@ -1849,20 +1912,22 @@ static void launch_density_g(const DevState& D, int block, int remap, hipStream_
     while (block > 64 && density_lds_bytes(W, block / G, ring) > 65536) block /= 2;
     const size_t lds = (size_t)density_lds_bytes(W, block / G, ring);
     const int nblk = (int)grid_of(D.P * G, block);
-    SPHE_PREC_SWITCH(D.precision,
-                     (klaunch_lds(k_density<float, true, G>, nblk, block, lds, st, e0, e1, D, nblk, remap)),
-                     (klaunch_lds(k_density<float, false, G>, nblk, block, lds, st, e0, e1, D, nblk, remap)),
-                     (klaunch_lds(k_density<double, false, G>, nblk, block, lds, st, e0, e1, D, nblk, remap)));
+#define CALL(T, SM) klaunch_lds(k_density<T, SM, G>, nblk, block, lds, st, e0, e1, D, nblk, remap)
+    SPHE_PREC_SWITCH(D.precision, CALL);
+#undef CALL
 }
 
 template <int G>
 static void launch_force_g(const DevState& D, int block, int remap, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     const int nblk = (int)grid_of(D.P * G, block);
-    SPHE_PREC_SWITCH(D.precision,
-                     (klaunch(k_force_f32<true, G>, nblk, block, st, e0, e1, D, nblk, remap)),
-                     (klaunch(k_force_f32<false, G>, nblk, block, st, e0, e1, D, nblk, remap)),
-                     (klaunch(k_force_f64<G>, nblk, block, st, e0, e1, D, nblk, remap)));
+    if (D.precision == 2) {
+        klaunch(k_force_f64<G>, nblk, block, st, e0, e1, D, nblk, remap);
+        return;
+    }
+#define CALL(T, SM) klaunch(k_force_f32<SM, G>, nblk, block, st, e0, e1, D, nblk, remap)
+    SPHE_PREC_SWITCH(D.precision, CALL);
+#undef CALL
 }
 
 void sphe_launch_density(const DevState& D, int block, int remap, int lanes, hipStream_t st, hipEvent_t e0,
@@ -1916,6 +1981,11 @@ void sphe_launch_two_phase(const DevState& D, hipStream_t st, hipEvent_t e0, hip
 
 void sphe_launch_clear_status(const DevState& D, hipStream_t st) { k_clear_status<<<1, 64, 0, st>>>(D.status); }
 
+void sphe_launch_publish(const DevState& D, uint32_t* host_word, hipStream_t st)
+{
+    k_publish<<<1, 64, 0, st>>>(D.ovf, host_word);
+}
+
 void sphe_launch_terrain_mesh(const float* H, int nx, int nz, float cell, float ox, float oz, float* out,
                               hipStream_t st)
 {
@@ -1925,31 +1995,42 @@ void sphe_launch_terrain_mesh(const float* H, int nx, int nz, float cell, float 
 }
 
 template <int G>
-static void launch_inspect_g(const DevState& D, int64_t slot, double* out, hipStream_t st)
+static void launch_inspect_g(const DevState& D, const int32_t* slotp, double* out, hipStream_t st)
 {
-    SPHE_PREC_SWITCH(D.precision,
-                     (k_inspect<float, true, G><<<1, 64, 0, st>>>(D, slot, out)),
-                     (k_inspect<float, false, G><<<1, 64, 0, st>>>(D, slot, out)),
-                     (k_inspect<double, false, G><<<1, 64, 0, st>>>(D, slot, out)));
+#define CALL(T, SM) k_inspect<T, SM, G><<<1, 64, 0, st>>>(D, slotp, out)
+    SPHE_PREC_SWITCH(D.precision, CALL);
+#undef CALL
 }
 
-void sphe_launch_inspect(const DevState& D, int64_t slot, int lanes, double* out, hipStream_t st)
+void sphe_launch_inspect(const DevState& D, const int32_t* slotp, int lanes, double* out, hipStream_t st)
 {
     switch (lanes) {
-    case 1: launch_inspect_g<1>(D, slot, out, st); break;
-    case 2: launch_inspect_g<2>(D, slot, out, st); break;
-    case 8: launch_inspect_g<8>(D, slot, out, st); break;
-    default: launch_inspect_g<4>(D, slot, out, st); break;
+    case 1: launch_inspect_g<1>(D, slotp, out, st); break;
+    case 2: launch_inspect_g<2>(D, slotp, out, st); break;
+    case 8: launch_inspect_g<8>(D, slotp, out, st); break;
+    default: launch_inspect_g<4>(D, slotp, out, st); break;
     }
+}
+
+void sphe_launch_idmap(const DevState& D, int32_t* map, int64_t len, hipStream_t st)
+{
+    if (len > 0) k_fill_i32<<<grid_of(len, 256), 256, 0, st>>>(map, len, INT32_MAX);
+    if (D.P > 0 && len > 0) k_idmap<<<grid_of(D.P, 256), 256, 0, st>>>(D, map);
+}
+
+void sphe_launch_find_id(const DevState& D, int scene, int32_t id, int32_t* slot_out, hipStream_t st)
+{
+    k_fill_i32<<<1, 64, 0, st>>>(slot_out, 1, INT32_MAX);
+    if (D.P > 0) k_find_id<<<grid_of(D.P, 256), 256, 0, st>>>(D, scene, id, slot_out);
 }
 
 template <int G>
 static void launch_accel_g(const DevState& D, int scene, double* out, hipStream_t st)
 {
     const unsigned nb = grid_of(D.P * G, 256);
-    SPHE_PREC_SWITCH(D.precision, (k_accel<float, true, G><<<nb, 256, 0, st>>>(D, scene, out)),
-                     (k_accel<float, false, G><<<nb, 256, 0, st>>>(D, scene, out)),
-                     (k_accel<double, false, G><<<nb, 256, 0, st>>>(D, scene, out)));
+#define CALL(T, SM) k_accel<T, SM, G><<<nb, 256, 0, st>>>(D, scene, out)
+    SPHE_PREC_SWITCH(D.precision, CALL);
+#undef CALL
 }
 
 void sphe_launch_accel(const DevState& D, int scene, int lanes, double* out, hipStream_t st)

@@ -25,9 +25,17 @@ void sphe_launch_force(const DevState& D, int block, int remap, int lanes, hipSt
 void sphe_launch_heights(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void sphe_launch_two_phase(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void sphe_launch_clear_status(const DevState& D, hipStream_t st);
+// Copies the list-overflow word (D.ovf) to a host word mapped for the device.
+void sphe_launch_publish(const DevState& D, uint32_t* host_word, hipStream_t st);
 void sphe_launch_terrain_mesh(const float* H, int nx, int nz, float cell, float ox, float oz, float* out,
                               hipStream_t st);
-void sphe_launch_inspect(const DevState& D, int64_t slot, int lanes, double* out, hipStream_t st);
+// The 112-byte record's values of the particle at sorted slot *slotp (device
+// memory) into out[0 .. 28]; out[22] = -1 when *slotp names no slot.
+void sphe_launch_inspect(const DevState& D, const int32_t* slotp, int lanes, double* out, hipStream_t st);
+// id -> sorted slot map of the last step for the scenes that have one.
+void sphe_launch_idmap(const DevState& D, int32_t* map, int64_t len, hipStream_t st);
+// Sorted slot of one particle id of a scene by a search over its slots.
+void sphe_launch_find_id(const DevState& D, int scene, int32_t id, int32_t* slot_out, hipStream_t st);
 void sphe_launch_accel(const DevState& D, int scene, int lanes, double* out, hipStream_t st);
 void sphe_launch_neighbours(const DevState& D, int scene, const int64_t* off, int32_t* counts, int32_t* idx,
                             hipStream_t st);

@@ -17,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -49,6 +50,7 @@ struct Scene {
     // layout (valid while the context layout is)
     int64_t p_begin = 0, p_cap = 0, cell_off = 0, ncell = 0, t_off = 0, tcap = 0;
     int dims[3] = {0, 0, 0};
+    int64_t idmap_off = -1, idmap_len = 0;  // the scene's part of the id -> slot map (-1: none)
 };
 
 struct ProfRec {
@@ -63,7 +65,9 @@ struct sphe_ctx {
     int device = 0;
     std::string err;
     hipStream_t stream = nullptr;
-    std::vector<Scene> scenes;
+    // a deque: adding a scene never moves the others, so the parameter
+    // storage sphe_param_ptr hands out stays where it is
+    std::deque<Scene> scenes;
 
     bool layout_valid = false;
     bool device_valid = false;  // device holds the latest state
@@ -78,9 +82,14 @@ struct sphe_ctx {
     bool table_uploaded = false;
 
     bool use_graph = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    // captured step sequences by length (steps per graph launch)
+    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
     std::string graph_sig;
+    // the list-overflow word as the last step sequence published it: host
+    // memory mapped for the device (h_ovf_dev), written by a kernel at the
+    // end of every sequence and read by the host without a stream wait
+    uint32_t* h_ovf = nullptr;
+    uint32_t* h_ovf_dev = nullptr;
 
     bool profiling = false;
     std::vector<ProfRec> pending;
@@ -99,6 +108,16 @@ struct sphe_ctx {
     double* d_mass_part = nullptr;
     double* d_mass_out = nullptr;
     double* d_inspect = nullptr;
+    // id -> sorted slot map of the last step (sphe_get_particle), rebuilt on
+    // the first inspection after a step; d_slot: the slot of a searched id
+    int32_t* d_idmap = nullptr;
+    int64_t idmap_len = 0;
+    bool idmap_valid = false;
+    int32_t* d_slot = nullptr;
+    // lane count and slab count the last step ran with: its inspection
+    // repeats its sums in the same order whatever was set since
+    int step_lanes = 4;
+    int step_sub_log2 = 2;
     float* d_mesh = nullptr;  // terrain vertex stream (sphe_terrain_mesh*)
     size_t mesh_cap = 0;      // floats
 
@@ -134,6 +153,22 @@ int fail(sphe_ctx* c, int code, const char* fmt, ...)
     } while (0)
 
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// hipMalloc that leaves no error behind when it fails: the runtime keeps a
+// failed call's error as its last error, and the launch checks that follow
+// (hipGetLastError) would report it as a failure of their own.
+template <typename T>
+hipError_t try_malloc(T** p, size_t bytes)
+{
+    void* q = nullptr;
+    const hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        q = nullptr;
+    }
+    *p = (T*)q;
+    return e;
+}
 
 size_t real_size(int precision) { return precision == SPHE_FP64 ? sizeof(double) : sizeof(float); }
 
@@ -220,7 +255,7 @@ void derive(const Scene& s, SceneDev& d, int yz)
     f.band = p.band;
     f.up = (float)std::ldexp(1.0, s.fx_exp);
     f.dn = (float)std::ldexp(1.0, -s.fx_exp);
-    f.dmax = (float)std::ldexp(1.0, 44 - s.fx_exp);
+    f.dmax = (float)std::ldexp(1.0, 52 - s.fx_exp);
 
     // FP64 comparison path: the same formulas in double from the FP32 values
     SceneConsts<double>& g = d.d;
@@ -257,7 +292,7 @@ void derive(const Scene& s, SceneDev& d, int yz)
     g.band = (double)p.band;
     g.up = std::ldexp(1.0, s.fx_exp);
     g.dn = std::ldexp(1.0, -s.fx_exp);
-    g.dmax = std::ldexp(1.0, 44 - s.fx_exp);
+    g.dmax = std::ldexp(1.0, 52 - s.fx_exp);
 
     d.hq = (float)(std::ldexp(1.0, -s.fx_exp) / area);
     d.t_y0 = s.t_org[1];
@@ -272,6 +307,8 @@ void derive(const Scene& s, SceneDev& d, int yz)
     d.cell_off = (int32_t)s.cell_off;
     d.ncell = (int32_t)s.ncell;
     d.t_off = (int32_t)s.t_off;
+    d.idmap_off = (int32_t)s.idmap_off;
+    d.idmap_len = (int32_t)s.idmap_len;
 }
 
 // ---------------------------------------------------------------- lattice
@@ -329,11 +366,19 @@ void sort_by_id(Scene& s)
 
 // ------------------------------------------------------------- buffers
 
+void drop_graphs(sphe_ctx* c)
+{
+    for (auto& kv : c->graphs) {
+        if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+        if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+    }
+    c->graphs.clear();
+    c->graph_sig.clear();
+}
+
 void free_device(sphe_ctx* c)
 {
-    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
-    c->graph_sig.clear();
+    drop_graphs(c);
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
     c->D = DevState{};
@@ -343,13 +388,16 @@ void free_device(sphe_ctx* c)
     c->stepped = false;
     c->table_uploaded = false;
     c->heights_stale = false;
+    c->d_idmap = nullptr;
+    c->idmap_len = 0;
+    c->idmap_valid = false;
 }
 
 int dalloc(sphe_ctx* c, void** p, size_t bytes)
 {
     *p = nullptr;
     if (bytes == 0) bytes = 256;
-    hipError_t e = hipMalloc(p, bytes);
+    hipError_t e = try_malloc(p, bytes);
     if (e != hipSuccess) return fail(c, SPHE_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
     c->allocs.push_back(*p);
     return SPHE_OK;
@@ -382,6 +430,20 @@ int build_layout(sphe_ctx* c)
         s.t_off = TC;
         s.tcap = round_up((int64_t)s.tnx * s.tnz, SPHE_WAVE);
         TC += s.tcap;
+    }
+    // id -> slot map: one entry per id 0 .. max id of each scene whose ids
+    // are dense enough (max id below 2 n + 4096; ids are sorted ascending in
+    // the staging arrays); the others are searched on inspection
+    int64_t M = 0;
+    for (Scene& s : c->scenes) {
+        s.idmap_off = -1;
+        s.idmap_len = 0;
+        if (s.ids.empty()) continue;
+        const int64_t len = (int64_t)s.ids.back() + 1;
+        if (len > 2 * (int64_t)s.ids.size() + 4096 || M + len > INT32_MAX / 2) continue;
+        s.idmap_off = M;
+        s.idmap_len = len;
+        M += len;
     }
     // per-slot vectors are addressed with 32-bit byte offsets: each stays under
     // 4 GiB, with room for the density pass's positions a block past the end
@@ -442,6 +504,10 @@ int build_layout(sphe_ctx* c)
     DALLOC(c->d_mass_part, (size_t)sphe_mass_blocks() * 3 * 8);
     DALLOC(c->d_mass_out, 64);
     DALLOC(c->d_inspect, 64 * 8);
+    DALLOC(c->d_slot, 64);
+    if (M > 0) DALLOC(c->d_idmap, (size_t)M * 4);
+    c->idmap_len = M;
+    c->idmap_valid = false;
     SceneDev* sc;
     DALLOC(sc, sizeof(SceneDev) * std::max<size_t>(1, c->scenes.size()));
     D.scenes = sc;
@@ -470,6 +536,7 @@ int build_layout(sphe_ctx* c)
     HIPCHK(c, hipMemsetAsync(D.ptop, 0, 64, c->stream));
     if (TC > 0) HIPCHK(c, hipMemsetAsync(D.vol, 0, (size_t)TC * 8, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    *(volatile uint32_t*)c->h_ovf = 0u;  // no overflow published for the new lists
     c->layout_valid = true;
     c->device_valid = false;
     return SPHE_OK;
@@ -760,20 +827,35 @@ std::string plan_signature(const sphe_ctx* c)
     return b;
 }
 
-int ensure_graph(sphe_ctx* c)
+// The executable graph of a sequence of k steps (captured on first use; all
+// sequences are dropped when the launch plan changes).  The sequence ends by
+// publishing the list-overflow word to the host.
+int ensure_graph(sphe_ctx* c, int k, hipGraphExec_t* exec)
 {
     const std::string sig = plan_signature(c);
-    if (c->gexec && sig == c->graph_sig) return SPHE_OK;
-    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    if (sig != c->graph_sig) {
+        drop_graphs(c);
+        c->graph_sig = sig;
+    }
+    auto it = c->graphs.find(k);
+    if (it != c->graphs.end()) {
+        *exec = it->second.second;
+        return SPHE_OK;
+    }
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    launch_step(c);
+    for (int i = 0; i < k; ++i) launch_step(c);
+    sphe_launch_publish(c->D, c->h_ovf_dev, c->stream);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (e != hipSuccess) return fail(c, SPHE_EHIP, "graph capture: %s", hipGetErrorString(e));
-    c->graph = g;
-    HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
-    c->graph_sig = sig;
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(c, SPHE_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    c->graphs[k] = std::make_pair(g, x);
+    *exec = x;
     return SPHE_OK;
 }
 
@@ -804,8 +886,9 @@ int resize_lists(sphe_ctx* c, int maxn)
     void* p = nullptr;
     void* q = nullptr;
     const int64_t pool = pool_entries(c->D.P, maxn);
-    if (hipMalloc(&p, (size_t)c->D.P * (size_t)maxn * 4) != hipSuccess) return SPHE_OK;  // lists as they were
-    if (hipMalloc(&q, (size_t)pool * 4) != hipSuccess) {
+    // on a failed allocation the old lists stay
+    if (try_malloc(&p, (size_t)c->D.P * (size_t)maxn * 4) != hipSuccess) return SPHE_OK;
+    if (try_malloc(&q, (size_t)pool * 4) != hipSuccess) {
         (void)hipFree(p);
         return SPHE_OK;
     }
@@ -826,28 +909,36 @@ int resize_lists(sphe_ctx* c, int maxn)
     return SPHE_OK;
 }
 
+// Grows the lists when a step sequence published a count above the
+// capacity.  The published word is read without waiting for the stream: it
+// may lag by the sequences still in flight, which only delays the growth
+// (an overflowing particle meanwhile continues in the pool or walks its
+// stencil, with the same result).
 int grow_lists(sphe_ctx* c)
 {
-    uint32_t need = 0;
-    HIPCHK(c, hipMemcpyAsync(&need, c->D.ovf, 4, hipMemcpyDeviceToHost, c->stream));
+    const uint32_t seen = *(volatile uint32_t*)c->h_ovf;
+    if (seen <= (uint32_t)c->D.maxn) return SPHE_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (need <= (uint32_t)c->D.maxn) return SPHE_OK;
+    uint32_t need = 0;
+    HIPCHK(c, hipMemcpy(&need, c->D.ovf, 4, hipMemcpyDeviceToHost));
+    need = std::max(need, seen);
     const int step = 128;
     const int r = resize_lists(c, (int)(((int64_t)need * 3 / 2 + step - 1) / step * step));
     if (r) return r;
     HIPCHK(c, hipMemsetAsync(c->D.ovf, 0, 4, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *(volatile uint32_t*)c->h_ovf = 0u;
     return SPHE_OK;
 }
+
+// Steps per captured graph launch of a long call.
+const int kSeqSteps = 32;
 
 int do_steps(sphe_ctx* c, int nsteps)
 {
     int r = prepare(c);
     if (r) return r;
     if (c->D.P == 0) return SPHE_OK;
-    if (c->stepped) {
-        r = grow_lists(c);
-        if (r) return r;
-    }
     sphe_launch_clear_status(c->D, c->stream);
     if (!c->counted) {
         r = recount(c);
@@ -855,28 +946,31 @@ int do_steps(sphe_ctx* c, int nsteps)
     }
     HIPCHK(c, hipGetLastError());
     const bool graph = c->use_graph && !c->profiling;
-    // A long call checks the list overflow flag every kList steps (one
-    // stream synchronisation each), so a state that compresses during the
-    // call does not keep walking its stencil to the end of the call.
-    const int kList = 16;
-    for (int k0 = 0; k0 < nsteps; k0 += kList) {
-        if (k0 > 0) {
-            r = grow_lists(c);
-            if (r) return r;
-        }
+    // Graph mode: sequences of kSeqSteps steps in one graph launch, the rest
+    // one step per launch; direct launches (profiling, or graphs off) one
+    // step at a time.  Before each launch the host looks at the overflow word
+    // the last finished sequence published.
+    for (int done = 0; done < nsteps;) {
+        r = grow_lists(c);
+        if (r) return r;
+        const int k = graph && nsteps - done >= kSeqSteps ? kSeqSteps : 1;
         if (graph) {
-            r = ensure_graph(c);
+            hipGraphExec_t x = nullptr;
+            r = ensure_graph(c, k, &x);
             if (r) return r;
-        }
-        const int k1 = nsteps - k0 < kList ? nsteps : k0 + kList;
-        for (int k = k0; k < k1; ++k) {
-            if (graph) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
-            else launch_step(c);
-            if (any_erosion(c, -1)) c->heights_stale = true;
+            HIPCHK(c, hipGraphLaunch(x, c->stream));
+        } else {
+            launch_step(c);
+            sphe_launch_publish(c->D, c->h_ovf_dev, c->stream);
         }
         HIPCHK(c, hipGetLastError());
+        done += k;
+        if (any_erosion(c, -1)) c->heights_stale = true;
         c->stepped = true;
     }
+    c->step_lanes = c->lanes;
+    c->step_sub_log2 = c->D.sub_log2;
+    c->idmap_valid = false;
     c->host_valid = false;
     return SPHE_OK;
 }
@@ -914,6 +1008,7 @@ int sort_current(sphe_ctx* c)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->counted = false;
     c->stepped = false;
+    c->idmap_valid = false;
     // the sorted buffers now hold the current state; keep the unsorted
     // state authoritative for the next step (its counts were consumed)
     return SPHE_OK;
@@ -993,7 +1088,7 @@ int sphe_create(int precision, sphe_ctx** out)
 {
     if (!out) return SPHE_EARG;
     *out = nullptr;
-    if (precision < SPHE_FP32 || precision > SPHE_FP64) return SPHE_EARG;
+    if (precision < SPHE_FP32 || precision > SPHE_FP32_F64ACC) return SPHE_EARG;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return SPHE_EHIP;
     sphe_ctx* c = new sphe_ctx();
@@ -1003,6 +1098,17 @@ int sphe_create(int precision, sphe_ctx** out)
         delete c;
         return SPHE_EHIP;
     }
+    void* h = nullptr;
+    if (hipHostMalloc(&h, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&c->h_ovf_dev, h, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        if (h) (void)hipHostFree(h);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return SPHE_EHIP;
+    }
+    c->h_ovf = (uint32_t*)h;
+    *(volatile uint32_t*)c->h_ovf = 0u;
     *out = c;
     return SPHE_OK;
 }
@@ -1018,6 +1124,7 @@ int sphe_destroy(sphe_ctx* c)
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     free_device(c);
     if (c->d_mesh) (void)hipFree(c->d_mesh);
+    if (c->h_ovf) (void)hipHostFree(c->h_ovf);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SPHE_OK;
@@ -1320,31 +1427,52 @@ int sphe_get_particle(sphe_ctx* c, int scene, int id, sphe_particle* out)
     SCENE_OR_FAIL(s, c, scene);
     if (!out) return fail(c, SPHE_EARG, "null output");
     std::memset(out, 0, sizeof(*out));
-    int r = sync_to_host(c);
-    if (r) return r;
-    const auto it = std::lower_bound(s->ids.begin(), s->ids.end(), id);
-    if (it == s->ids.end() || *it != id) return fail(c, SPHE_ERANGE, "no particle with id %d", id);
-    const size_t hi = (size_t)(it - s->ids.begin());
     out->Id = id;
     out->NeighbId = -1;
-    for (int a = 0; a < 3; ++a) {
-        out->Position[a] = (float)s->pos[3 * hi + a];
-        out->Velocity[a] = (float)s->vel[3 * hi + a];
+    if (!c->stepped || !c->device_valid) {
+        // no step since the last edit: the staging arrays hold the state, and
+        // there are no forces yet
+        int r = sync_to_host(c);
+        if (r) return r;
+        const auto it = std::lower_bound(s->ids.begin(), s->ids.end(), id);
+        if (it == s->ids.end() || *it != id) return fail(c, SPHE_ERANGE, "no particle with id %d", id);
+        const size_t hi = (size_t)(it - s->ids.begin());
+        for (int a = 0; a < 3; ++a) {
+            out->Position[a] = (float)s->pos[3 * hi + a];
+            out->Velocity[a] = (float)s->vel[3 * hi + a];
+        }
+        return SPHE_OK;
     }
-    if (!c->stepped) return SPHE_OK;
-    // slot of the id in the last step's sorted buffers
-    const size_t n = s->ids.size();
-    std::vector<int32_t> sid(n);
-    HIPCHK(c, hipMemcpy(sid.data(), c->D.s_id + s->p_begin, n * 4, hipMemcpyDeviceToHost));
-    size_t slot = n;
-    for (size_t i = 0; i < n; ++i)
-        if (sid[i] == id) { slot = i; break; }
-    if (slot == n) return fail(c, SPHE_ERANGE, "id %d not in the last step", id);
-    sphe_launch_inspect(c->D, s->p_begin + (int64_t)slot, c->lanes, c->d_inspect, c->stream);
+    if (id < 0) return fail(c, SPHE_ERANGE, "no particle with id %d", id);
+    // the slot of the id in the last step's sorted buffers: from the id map
+    // (rebuilt once per step, on the first inspection) or by a search
+    const int32_t* slotp = nullptr;
+    if (s->idmap_off >= 0) {
+        if (id >= s->idmap_len) return fail(c, SPHE_ERANGE, "no particle with id %d", id);
+        if (!c->idmap_valid) {
+            sphe_launch_idmap(c->D, c->d_idmap, c->idmap_len, c->stream);
+            HIPCHK(c, hipGetLastError());
+            c->idmap_valid = true;
+        }
+        slotp = c->d_idmap + s->idmap_off + id;
+    } else {
+        sphe_launch_find_id(c->D, scene, id, c->d_slot, c->stream);
+        HIPCHK(c, hipGetLastError());
+        slotp = c->d_slot;
+    }
+    // the sums in the order the step took them
+    DevState D = c->D;
+    D.sub_log2 = c->step_sub_log2;
+    sphe_launch_inspect(D, slotp, c->step_lanes, c->d_inspect, c->stream);
     HIPCHK(c, hipGetLastError());
-    double v[22];
+    double v[29];
     HIPCHK(c, hipMemcpyAsync(v, c->d_inspect, sizeof(v), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (v[22] < 0.0) return fail(c, SPHE_ERANGE, "no particle with id %d", id);
+    for (int a = 0; a < 3; ++a) {
+        out->Position[a] = (float)v[23 + a];
+        out->Velocity[a] = (float)v[26 + a];
+    }
     out->Density = (float)v[0];
     out->Pressure = (float)v[1];
     for (int a = 0; a < 3; ++a) {
@@ -1383,14 +1511,18 @@ int sphe_terrain_load_u8(sphe_ctx* c, int scene, const uint8_t* img, int w, int 
     if (!img || w <= 0 || h <= 0) return fail(c, SPHE_EARG, "bad image");
     int r = begin_host_edit(c);
     if (r) return r;
-    // Grid::GetHeightfieldAt(x, y) = map[dimY * x + y] (grid.h:104-107):
-    // terrain cell (x, z) reads map[h * x + z].
-    s->tnx = w;
-    s->tnz = h;
+    // The image is row-major, w pixels per row and h rows, as stbi_load
+    // returns it (main.cpp:88-92), and Grid copies it unchanged
+    // (grid.h:98-102).  Grid::GetHeightfieldAt(x, y) = map[dimY * x + y]
+    // (grid.h:104-107) with dimY the row length: x picks the image row and y
+    // the column.  So terrain cell (x, z) is pixel (row x, column z), and the
+    // terrain has h cells along x and w along z.
+    s->tnx = h;
+    s->tnz = w;
     if (!(s->t_cell > 0.0f)) s->t_cell = 1.0f;
     std::vector<float> hts((size_t)w * h);
-    for (int z = 0; z < h; ++z)
-        for (int x = 0; x < w; ++x) hts[(size_t)z * w + x] = s->t_org[1] + scale * (float)img[(size_t)h * x + z];
+    for (int z = 0; z < w; ++z)
+        for (int x = 0; x < h; ++x) hts[(size_t)z * h + x] = s->t_org[1] + scale * (float)img[(size_t)w * x + z];
     return set_terrain_heights(c, *s, hts.data());
 }
 
@@ -1426,8 +1558,11 @@ int sphe_set_terrain_volume_fx(sphe_ctx* c, int scene, const int64_t* vol, size_
     SCENE_OR_FAIL(s, c, scene);
     const size_t nt = (size_t)s->tnx * s->tnz;
     if (!vol || nt == 0 || count != nt) return fail(c, SPHE_EARG, "expected %zu cell volumes", nt);
+    // erosion is not capped at a cell's volume, so a cell can go below the
+    // terrain base (a negative volume): any value in the fixed-point range
     for (size_t t = 0; t < nt; ++t)
-        if (vol[t] < 0 || vol[t] > ((int64_t)1 << 56)) return fail(c, SPHE_ERANGE, "cell %zu volume out of range", t);
+        if (vol[t] < -((int64_t)1 << 56) || vol[t] > ((int64_t)1 << 56))
+            return fail(c, SPHE_ERANGE, "cell %zu volume out of range", t);
     int r = begin_host_edit(c);
     if (r) return r;
     std::memcpy(s->vol.data(), vol, nt * 8);
@@ -1461,7 +1596,7 @@ int mesh_on_device(sphe_ctx* c, const Scene* s, float** out)
         if (c->d_mesh) (void)hipFree(c->d_mesh);
         c->d_mesh = nullptr;
         c->mesh_cap = 0;
-        HIPCHK(c, hipMalloc(&c->d_mesh, nv * 4));
+        HIPCHK(c, try_malloc(&c->d_mesh, nv * 4));
         c->mesh_cap = nv;
     }
     sphe_launch_terrain_mesh(c->D.height + s->t_off, s->tnx, s->tnz, s->t_cell, s->t_org[0], s->t_org[2],
@@ -1925,7 +2060,7 @@ int sphe_debug_cell_keys(sphe_ctx* c, int scene, int32_t* keys)
     const size_t n = s->ids.size();
     if (n == 0) return SPHE_OK;
     int32_t* dk = nullptr;
-    HIPCHK(c, hipMalloc(&dk, (size_t)s->p_cap * 4));
+    HIPCHK(c, try_malloc(&dk, (size_t)s->p_cap * 4));
     sphe_launch_keys(c->D, scene, dk, c->stream);
     std::vector<int32_t> k(n), id(n);
     hipError_t e1 = hipMemcpyAsync(k.data(), dk, n * 4, hipMemcpyDeviceToHost, c->stream);
@@ -1966,7 +2101,7 @@ int sphe_debug_neighbours(sphe_ctx* c, int scene, int32_t* counts, const int64_t
     std::vector<int32_t> sid(n), cnt(n);
     HIPCHK(c, hipMemcpy(sid.data(), c->D.s_id + s->p_begin, n * 4, hipMemcpyDeviceToHost));
     int32_t* dcnt = nullptr;
-    HIPCHK(c, hipMalloc(&dcnt, n * 4));
+    HIPCHK(c, try_malloc(&dcnt, n * 4));
     sphe_launch_neighbours(c->D, scene, nullptr, dcnt, nullptr, c->stream);
     hipError_t e = hipMemcpyAsync(cnt.data(), dcnt, n * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1986,8 +2121,8 @@ int sphe_debug_neighbours(sphe_ctx* c, int scene, int32_t* counts, const int64_t
     for (size_t i = 0; i < n; ++i) soff[i + 1] = soff[i] + cnt[i];
     int64_t* doff = nullptr;
     int32_t* didx = nullptr;
-    e = hipMalloc(&doff, (n + 1) * 8);
-    if (e == hipSuccess) e = hipMalloc(&didx, (size_t)std::max<int64_t>(1, soff[n]) * 4);
+    e = try_malloc(&doff, (n + 1) * 8);
+    if (e == hipSuccess) e = try_malloc(&didx, (size_t)std::max<int64_t>(1, soff[n]) * 4);
     if (e == hipSuccess) e = hipMemcpy(doff, soff.data(), (n + 1) * 8, hipMemcpyHostToDevice);
     std::vector<int32_t> hidx((size_t)soff[n]);
     if (e == hipSuccess) {
@@ -2032,7 +2167,7 @@ int sphe_debug_accel(sphe_ctx* c, int scene, double* acc)
     const size_t n = s->ids.size();
     if (n == 0) return SPHE_OK;
     double* d = nullptr;
-    HIPCHK(c, hipMalloc(&d, n * 3 * 8));
+    HIPCHK(c, try_malloc(&d, n * 3 * 8));
     sphe_launch_accel(c->D, scene, c->lanes, d, c->stream);
     hipError_t e = hipGetLastError();
     std::vector<double> v(n * 3);
@@ -2055,8 +2190,8 @@ int sphe_debug_copy(sphe_ctx* c, size_t bytes, int iters, double* avg_ms)
     if (!c || bytes == 0 || bytes % 16 || iters < 1) return fail(c, SPHE_EARG, "bytes must be a positive multiple of 16");
     void* a = nullptr;
     void* b = nullptr;
-    HIPCHK(c, hipMalloc(&a, bytes));
-    hipError_t e = hipMalloc(&b, bytes);
+    HIPCHK(c, try_malloc(&a, bytes));
+    hipError_t e = try_malloc(&b, bytes);
     if (e != hipSuccess) {
         (void)hipFree(a);
         return fail(c, SPHE_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));

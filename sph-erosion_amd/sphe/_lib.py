@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "libsphe.so")
 SPHE_OK = 0
 ERRORS = {-1: "SPHE_EARG", -2: "SPHE_EHIP", -3: "SPHE_ESTATE", -4: "SPHE_ENOMEM", -5: "SPHE_ERANGE"}
 
-FP32, FP32_NAIVE, FP64 = 0, 1, 2
+FP32, FP32_NAIVE, FP64, FP32_F64ACC = 0, 1, 2, 3
 EROSION_OFF, EROSION_ATOMIC, EROSION_TWO_PHASE = 0, 1, 2
 P_MASS, P_VISC, P_SURF_TEN, P_P0, P_GRAV = 0, 1, 2, 3, 4
 BUF_POS, BUF_VEL, BUF_DENSITY, BUF_MASS, BUF_PRESS, BUF_SED, BUF_ID = range(7)

@@ -204,6 +204,23 @@ class Context:
         self._chk(lib().sphe_terrain_volume_fx(self._p, scene, _ptr(out), out.size), "terrain_volume_fx")
         return out
 
+    def set_terrain(self, scene, terrain):
+        """Attach, replace or (None) remove a scene's heightfield; heights may be
+        None for a flat terrain at the origin's y."""
+        if terrain is None:
+            self._chk(lib().sphe_set_terrain(self._p, scene, None), "set_terrain")
+            return
+        td = TerrainDesc()
+        td.nx, td.nz = int(terrain["nx"]), int(terrain["nz"])
+        for i in range(3):
+            td.origin[i] = float(terrain["origin"][i])
+        td.cell = float(terrain["cell"])
+        if terrain.get("heights") is not None:
+            hts = np.ascontiguousarray(terrain["heights"], dtype=np.float32)
+            self._terrains.append(hts)
+            td.heights = hts.ctypes.data
+        self._chk(lib().sphe_set_terrain(self._p, scene, C.byref(td)), "set_terrain")
+
     def set_terrain_volume_fx(self, scene, vol):
         """Restore fixed-point cell volumes (a checkpoint's terrain)."""
         v = np.ascontiguousarray(vol, dtype=np.int64).ravel()
@@ -215,8 +232,10 @@ class Context:
         return e.value
 
     def terrain_load_u8(self, scene, img, scale):
+        """Grid::LoadHeightfield of an image array of shape (rows, columns):
+        terrain cell (x, z) = img[x, z] (GetHeightfieldAt, grid.h:104-107)."""
         img = np.ascontiguousarray(img, dtype=np.uint8)
-        w, h = img.shape[0], img.shape[1]
+        h, w = img.shape[0], img.shape[1]
         self._chk(lib().sphe_terrain_load_u8(self._p, scene, _ptr(img), w, h, float(scale)), "terrain_load_u8")
 
     def terrain_mesh(self, scene):

@@ -95,3 +95,23 @@ class FluidSystemSPH:
 
     def NumParticles(self):
         return self.ctx.num_particles(self.scene)
+
+    # terrain coupling (Grid::LoadHeightfield / UpdateGrid / GetSurfaceParts /
+    # GetIndices, grid.h:98-176), as the C++ drop-in header exposes it
+    def LoadHeightfield(self, img, scale, cell, origin, erosion=None):
+        """img: uint8 array (rows, columns); terrain cell (x, z) takes
+        GetHeightfieldAt(x, z) = img[x, z] (grid.h:104-107)."""
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        p = self.ctx.get_params(self.scene)
+        if erosion is not None:
+            p.erosion = erosion
+            self.ctx.set_params(self.scene, p)
+        self.ctx.set_terrain(self.scene, dict(nx=img.shape[0], nz=img.shape[1], origin=tuple(origin),
+                                              cell=float(cell), heights=None))
+        self.ctx.terrain_load_u8(self.scene, img, scale)
+
+    def GetSurfaceParts(self):
+        return self.ctx.terrain_mesh(self.scene)[0].reshape(-1)
+
+    def GetIndices(self):
+        return self.ctx.terrain_mesh(self.scene)[1]
