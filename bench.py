@@ -1,22 +1,36 @@
 #!/usr/bin/env python3
 """Benchmark of the SPH-Erosion step on one MI355X.
 
-Workload (default): the single large scene, 262,144 particles in a column
-(s = 1/256, h = 2.15 s) dropped onto a 512 x 512 value-noise heightfield
-(6 octaves, amplitude 0.3), dt = 5e-4, erosion on (atomic), FP32 with
-compensated sums.  A step is one pass of the whole hot path (sort, density,
-forces + integrate + collisions + erosion, terrain heights) over the scene.
+Workload (default): the long-horizon batched production configuration,
+1,024 dam-break scenes (N log-uniform in [1,024, 16,384], terrains 64..256
+cells square) plus one 8,388,608-particle layer on a 4096^2 value-noise
+terrain, 13,954,298 particles in one context, dt = 5e-4, erosion on
+(atomic), FP32 with Kahan-compensated density and force sums.  A step is one
+pass of the whole hot path (sort, density, forces + integrate + collisions +
+erosion, terrain heights) over every scene.
 
-Prints one JSON line: particle-steps/s over the timed region (K steps as
-the library runs them by default, one HIP graph per step, inputs resident in
-HBM); the same K steps replayed from a saved state kernel by kernel, with
-each kernel's start and end in HIP events on the library's stream, for the
-per-kernel times and the HBM roofline of the dominant kernel (the replay
-must end in a bit-identical state); the density error against the
-library's FP64 path on the final state; the accuracy protocol (one step in
-FP32 compensated, FP32 plain and FP64 from a shared state); the plain FP64
-GPU path over the same steps; and the reference step timed on the host's
-cores (the CPU oracle's restatement of it, the only use of the oracle here).
+The run is a window of the workload's trajectory: `--settle` steps first
+(untimed; by default 1,000 of the 2,000-step run, where the water is on the
+terrain and eroding), then `--warmup` steps, then the K timed steps.
+
+Prints one JSON line:
+  * value: particle-steps/s over the K timed steps as the library runs them by
+    default (captured step sequences, inputs resident in HBM);
+  * the same K steps replayed from a saved state kernel by kernel, each
+    kernel's start and end in HIP events on the library's stream: per-kernel
+    times, the HBM roofline of the dominant kernel and of the force pass (the
+    replay must end in a bit-identical state);
+  * terrain contact in the window: particles that exchanged sediment with the
+    terrain, terrain cells that changed;
+  * water/sediment bookkeeping per scene over the window (FP64 reductions,
+    exact fixed-point units);
+  * density error against the library's FP64 path on the final state; the
+    accuracy protocol (one step in each FP32 mode and FP64 from a shared
+    state); the plain FP64 GPU path over the same window; and the reference
+    step on the host's cores (the CPU oracle's restatement of it, the only
+    use of the oracle here).
+Exits with status 3 when the step reported a non-finite state or a particle it
+could not place.
 """
 import argparse
 import json
@@ -32,7 +46,10 @@ for p in (os.path.join(ROOT, "sph-erosion_amd"), os.path.join(ROOT, "oracle")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
+SIMDS = 256 * 4        # MI355X: 256 CUs, 4 SIMDs each
+CLOCK_GHZ = 2.4        # MI355X peak engine clock
+COUNTERS = os.path.join(ROOT, "profiles", "counters.json")
+SETTLE = {"cfg1": 100, "cfg2": 400, "cfg3": 500, "cfg4": 600, "cfg5": 1000, "cfg5_share": 1000}
 
 
 def log(msg):
@@ -41,21 +58,22 @@ def log(msg):
 
 def make_workload(name):
     from sphe import scenes
-    if name == "cfg2":
-        return [scenes.cfg2()], "column 262,144 particles on 512^2 value-noise terrain (6 oct, amp 0.3), h=2.15/256, dt=5e-4, erosion atomic"
     if name == "cfg1":
-        return [scenes.cfg1()], "dam break 4,096 particles on 64^2 terrain, h=0.0457, dt=1e-3, erosion atomic"
+        return [scenes.cfg1()], "dam break 4,096 particles on 64^2 terrain, h=0.0457, dt=1e-3, erosion atomic", 200
+    if name == "cfg2":
+        return ([scenes.cfg2()], "column 262,144 particles on 512^2 value-noise terrain (6 oct, amp 0.3), "
+                "h=2.15/256, dt=5e-4, erosion atomic", 500)
     if name == "cfg3":
-        return scenes.cfg3(), "256 batched dam-break scenes, N log-uniform [2048, 65536], T in {64..256}"
+        return scenes.cfg3(), "256 batched dam-break scenes, N log-uniform [2048, 65536], T in {64..256}", 300
     if name == "cfg4":
-        return [scenes.cfg4()], "layer of 4,194,304 particles over ridged 2048^2 terrain, dt=5e-4"
+        return [scenes.cfg4()], "layer of 4,194,304 particles over ridged 2048^2 terrain, dt=5e-4", 1000
     if name == "cfg5":
         return scenes.cfg5(), ("1,024 batched dam-break scenes, N log-uniform [1024, 16384], plus one "
-                               "8,388,608-particle layer on a 4096^2 terrain, dt=5e-4")
+                               "8,388,608-particle layer on a 4096^2 terrain, dt=5e-4, erosion atomic"), 2000
     if name == "cfg5_share":
         sc = scenes.cfg5_share()
         return [sc], (f"one GPU's x-slab of cfg5's 8,388,608-particle layer: {sc.n:,} particles "
-                      "(1/8 of the layer plus a one-cell ghost layer) on the 4096^2 terrain")
+                      "(1/8 of the layer plus a one-cell ghost layer) on the 4096^2 terrain"), 2000
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -74,47 +92,45 @@ def snapshot(ctx, sids, f32=False):
     return out
 
 
-def load_state(ctx, sids, states):
-    for s, st in zip(sids, states):
-        ctx.set_particles(s, st["pos"], vel=st["vel"], ids=st["ids"], sed=st["sed"])
-        if st["vol"] is not None:
-            ctx.set_terrain_volume_fx(s, st["vol"])
-
-
-def context_like(scs, precision, blk, rm):
+def context_from(scs, states, precision, tuning):
     import sphe
     c = sphe.Context(precision)
-    sids = [c.add_synthetic(s) for s in scs]
-    c.set_tuning(blk, rm)
+    sids = [c.add_scene(s.params, s.terrain) for s in scs]
+    for s, st in zip(sids, states):
+        c.set_particles(s, st["pos"], vel=st["vel"], ids=st["ids"], sed=st["sed"])
+        if st["vol"] is not None:
+            c.set_terrain_volume_fx(s, st["vol"])
+    c.set_tuning(tuning[0], tuning[1])
+    c.set_force_block(tuning[2])
     return c, sids
 
 
-def density_error(ctx, sids, scs, blk, rm):
+def density_error(ctx, sids, scs, end_state, tuning):
     """Max relative density error of the FP32 pass against the library's FP64
-    path on the same state (the state the timed region ended in)."""
+    path on the same state (the state the timed window ended in)."""
     import sphe
-    c64, s64 = context_like(scs, sphe.FP64, blk, rm)
-    load_state(c64, s64, snapshot(ctx, sids))
+    c64, s64 = context_from(scs, end_state, sphe.FP64, tuning)
     rho = np.concatenate([ctx.debug_density(s) for s in sids])
     ref = np.concatenate([c64.debug_density(s) for s in s64])
     c64.close()
     return float((np.abs(rho - ref) / ref).max())
 
 
-def accuracy_protocol(scs, blk, rm, warm=50):
-    """50 FP64 warm-up steps; the state rounded to FP32; from it, one step in
-    FP32 compensated, FP32 plain and FP64: density and acceleration of that
-    step, terrain height and sediment changes, each against FP64."""
+def accuracy_protocol(scs, start_state, tuning, warm=50):
+    """From the state the timed window started in: `warm` FP64 steps; that
+    state rounded to FP32; from it one step in FP64 and in each FP32 mode:
+    density and acceleration of the step, terrain height and sediment
+    changes, each against FP64."""
     import sphe
-    c64, s64 = context_like(scs, sphe.FP64, blk, rm)
+    c64, s64 = context_from(scs, start_state, sphe.FP64, tuning)
     c64.step(warm)
     c64.sync()
     st = snapshot(c64, s64, f32=True)
     c64.close()
     res = {}
-    for name, prec in (("fp64", sphe.FP64), ("fp32_compensated", sphe.FP32), ("fp32_plain", sphe.FP32_NAIVE)):
-        c, ss = context_like(scs, prec, blk, rm)
-        load_state(c, ss, st)
+    for name, prec in (("fp64", sphe.FP64), ("fp32_compensated", sphe.FP32),
+                       ("fp32_f64_force_sums", sphe.FP32_F64ACC), ("fp32_plain", sphe.FP32_NAIVE)):
+        c, ss = context_from(scs, st, prec, tuning)
         rho = np.concatenate([c.debug_density(s) for s in ss])
         acc = np.concatenate([c.debug_accel(s) for s in ss])
         h0 = [c.terrain_heights(s).astype(np.float64) for s in ss]
@@ -126,10 +142,10 @@ def accuracy_protocol(scs, blk, rm, warm=50):
         c.close()
     rho64, acc64, dh64, ds64 = res["fp64"]
     n64 = np.linalg.norm(acc64, axis=1)
-    out = dict(protocol=f"{warm} FP64 warm-up steps, state rounded to FP32, one step per precision from it; "
-                        "errors against the FP64 step",
-               particles=int(len(rho64)))
-    for name in ("fp32_compensated", "fp32_plain"):
+    out = dict(protocol=f"state at the window start, {warm} FP64 steps, rounded to FP32; one step per "
+                        "precision from it; errors against the FP64 step",
+               particles=int(len(rho64)), exchanging_particles=int((ds64 != 0).sum()))
+    for name in ("fp32_compensated", "fp32_f64_force_sums", "fp32_plain"):
         rho, acc, dh, ds = res[name]
         rr = np.abs(rho - rho64) / rho64
         ok = n64 > 0
@@ -145,46 +161,113 @@ def accuracy_protocol(scs, blk, rm, warm=50):
     return out
 
 
-def cpu_baseline(sc, target_s=12.0):
-    """The reference step (oracle ref32: all pairs, AoS, FP32, OpenMP over i)
-    on this host's cores, over a strided sample of i (each still over all j)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def ref_records(sc):
+    """The scene as the reference's particle records: its box is centred at 0,
+    so the unit box [0, 1]^3 shifts to [-0.5, 0.5]^3 (len 0.5)."""
     import oracle as O
-    n = sc.n
-    P = np.zeros(n, dtype=O.PARTICLE_DTYPE)
-    P["Id"] = np.arange(n)
-    # the reference box is centred at 0: shift the unit box to [-0.5, 0.5]
+    P = np.zeros(sc.n, dtype=O.PARTICLE_DTYPE)
+    P["Id"] = np.arange(sc.n)
     P["Position"] = (sc.pos - 0.5).astype(np.float32)
     P["Velocity"] = sc.vel.astype(np.float32)
     prm = O.default_params(dt=sc.params.dt)
     prm.h, prm.mass, prm.p0, prm.k = sc.params.h, sc.params.mass, sc.params.rest_density, sc.params.stiffness
     prm.visc, prm.surf_tens, prm.len = sc.params.visc, sc.params.surf_tens, 0.5
     prm.g[0], prm.g[1], prm.g[2] = sc.params.gravity[0], sc.params.gravity[1], sc.params.gravity[2]
-    stride = max(1, n // 256)
+    return P, prm
+
+
+def ref_time_per_i(sc, target_s):
+    """Seconds of the reference step per particle i (each i over all n j, three
+    passes) from a strided sample of i sized to about target_s seconds."""
+    import oracle as O
+    P, prm = ref_records(sc)
+    stride = max(1, sc.n // 64)
     t = time.perf_counter()
     m = O.ref32_step_sampled(P, prm, stride)
-    dt0 = time.perf_counter() - t
-    per_i = dt0 / max(m, 1)
-    want = int(target_s / max(per_i, 1e-9))
-    stride2 = max(1, n // max(want, 1))
+    per_i = (time.perf_counter() - t) / max(m, 1)
+    stride2 = max(1, sc.n // max(1, int(target_s / max(per_i, 1e-12))))
     t = time.perf_counter()
     m2 = O.ref32_step_sampled(P, prm, stride2)
-    dt1 = time.perf_counter() - t
-    return dict(value=m2 / dt1, unit="particle-steps/s", cores=O.ref32_threads(), kind="port",
-                sample=f"reference step (all pairs, 3 passes over all {n} j) for every {stride2}-th i "
-                       f"= {m2} particles, {dt1:.1f} s; particle-steps/s = particles processed / time")
+    return (time.perf_counter() - t) / max(m2, 1), m2, stride2
+
+
+def cpu_baseline(scs, target_s=12.0):
+    """The reference step (the oracle's ref32: all pairs, AoS, FP32, OpenMP over
+    i) on this host's cores.  The cost of one step of every scene is built
+    from samples and extrapolated: a scene above 100,000 particles from a
+    strided sample of its i (each i still over all j); the smaller scenes from
+    whole steps of up to 24 of them, scaled by n^2 (all-pairs cost)."""
+    import oracle as O
+    big = [s for s in scs if s.n > 100_000]
+    small = [s for s in scs if s.n <= 100_000]
+    total_t = 0.0
+    parts = []
+    budget = target_s / (2 if big and small else 1)
+    for sc in big:
+        per_i, m, stride = ref_time_per_i(sc, budget / len(big))
+        total_t += per_i * sc.n
+        parts.append(f"{sc.n:,}-particle scene: every {stride}-th i ({m:,} of them, each over all j)")
+    if small:
+        order = np.argsort([s.n for s in small])
+        pick = [small[i] for i in order[np.linspace(0, len(small) - 1, min(24, len(small))).astype(int)]]
+        spent, cost, n2 = 0.0, 0.0, 0.0
+        for sc in pick:
+            P, prm = ref_records(sc)
+            t = time.perf_counter()
+            O.ref32_step(P, prm, 1)
+            dt = time.perf_counter() - t
+            cost += dt
+            n2 += float(sc.n) ** 2
+            spent += dt
+            if spent > budget:
+                break
+        total_t += cost / n2 * sum(float(s.n) ** 2 for s in small)
+        parts.append(f"{len(small)} scenes of <= 100,000 particles: whole steps of {len(pick)} spread over "
+                     "their sizes, scaled by n^2")
+    N = sum(s.n for s in scs)
+    extrapolated = len(scs) > 1 or bool(big)
+    return dict(value=N / total_t, unit="particle-steps/s", cores=O.ref32_threads(), kind="port",
+                cpu_model=cpu_model(), extrapolated=extrapolated,
+                sample="reference step (all pairs, 3 passes over all j, FP32, AoS) timed on " + "; ".join(parts) +
+                       f"; one step of all {N:,} particles would take {total_t:.1f} s; "
+                       "particle-steps/s = particles / that time")
+
+
+def roofline(name, k, counters):
+    achieved = k["bytes"] / (k["avg_ms"] * 1e-3) / 1e9
+    c = counters.get(name, {})
+    out = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+               traffic=c.get("hbm_bytes"), kernel=name, bytes_per_launch=k["bytes"], avg_ms=k["avg_ms"])
+    if c.get("valu_insts") is not None:
+        # VALU issue: each wave64 VALU instruction occupies a 16-lane SIMD for
+        # 4 cycles (an upper bound for part of the mix), over the kernel time
+        out["valu_issue_frac"] = c["valu_insts"] * 4 / (SIMDS * CLOCK_GHZ * 1e9 * k["avg_ms"] * 1e-3)
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="cfg5")
+    ap.add_argument("--settle", type=int, default=None,
+                    help="untimed steps before the warm-up (default: per workload, into terrain contact)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--trace", type=int, default=0, help="steps to trace before the warmup (diagnostics)")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the timed region (for profiler runs): no statistics, accuracy, FP64 or CPU legs")
     ap.add_argument("--mode", default="step", choices=["step", "copy"],
@@ -203,8 +286,9 @@ def main():
         return
     if a.no_extras:
         a.no_cpu = a.no_fp64 = a.no_tune = True
+    settle = SETTLE.get(a.workload, 0) if a.settle is None else a.settle
     log(f"bench: {sphe.lib().sphe_version().decode()} workload {a.workload}")
-    scs, desc = make_workload(a.workload)
+    scs, desc, run_steps = make_workload(a.workload)
     N = sum(s.n for s in scs)
 
     ctx = sphe.Context(sphe.FP32)
@@ -214,36 +298,41 @@ def main():
         log(f"bench: autotuned block={blk} force_block={ctx.get_force_block()} xcd_remap={rm}")
     blk, rm = ctx.get_tuning()
     fblk = ctx.get_force_block()
-    if a.trace:
-        # ms/step and neighbour counts along the trajectory, 50-step chunks
-        for c in range(0, a.trace, 50):
-            t = time.perf_counter()
-            ctx.step(50)
-            ctx.sync()
-            ms = (time.perf_counter() - t) * 1e3 / 50
-            cnt = ctx.debug_neighbours(sids[0], csr=False)[0]
-            log(f"trace: steps {c + 50:5d} ms/step {ms:.3f} neighbours mean {cnt.mean():.1f} max {cnt.max()}")
+    tuning = (blk, rm, fblk)
+    for done in range(0, settle, 250):
+        ctx.step(min(250, settle - done))
+        ctx.sync()
+        log(f"bench: settled {min(done + 250, settle)} of {settle} steps")
     ctx.step(a.warmup)
     ctx.sync()
     ctx.save_state()
-    mass0 = [ctx.mass(s) for s in sids]
+    start_state = snapshot(ctx, sids) if not a.no_extras else None
+    sed0 = np.concatenate([ctx.sediment_fx(s) for s in sids])
+    vol0 = [ctx.terrain_volume_fx(s) for s in sids]
+    t0m, s0m, u0m = ctx.mass_all()
 
-    # timed region: K steps as the library runs them by default (each step
-    # one HIP graph launch on the library's stream), inputs resident in HBM
+    # timed window: K steps as the library runs them by default (captured
+    # step sequences on the library's stream), inputs resident in HBM
+    log(f"bench: timing {a.steps} steps")
     ctx.sync()
     t0 = time.perf_counter()
     ctx.step(a.steps)
     ctx.sync()
     elapsed = time.perf_counter() - t0
     status = ctx.status()
-    mass1 = [ctx.mass(s) for s in sids]
-    drift = max(abs((m1[0] + m1[1]) - (m0[0] + m0[1])) / (m0[0] + m0[1]) for m0, m1 in zip(mass0, mass1))
-    units_exact = all(m0[2] == m1[2] for m0, m1 in zip(mass0, mass1))
-    end_state = [ctx.state_f64(s) for s in sids]
+    t1m, s1m, u1m = ctx.mass_all()
+    tot0 = t0m + s0m
+    drift = np.abs((t1m + s1m) - tot0) / np.where(tot0 > 0, tot0, 1.0)
+    units_exact = bool(np.array_equal(u0m, u1m))
+    sed1 = np.concatenate([ctx.sediment_fx(s) for s in sids])
+    cells_changed = sum(int((ctx.terrain_volume_fx(s) != v).sum()) for s, v in zip(sids, vol0))
+    end_state = snapshot(ctx, sids)
 
     # the same K steps again from the saved state, kernel by kernel: each
     # kernel's start and end land in HIP events on the library's stream
+    log("bench: replaying the window kernel by kernel")
     ctx.restore_state()
+    ctx.mass_all()  # uploads the restored state before the clock starts
     ctx.profile(True)
     ctx.sync()
     tp = time.perf_counter()
@@ -255,63 +344,55 @@ def main():
         ms, cnt = ctx.profile_read(name)
         kernels[name] = dict(avg_ms=ms, launches=cnt, bytes=ctx.kernel_bytes(name) if name != "two_phase" else None)
     ctx.profile(False)
-    replay_identical = all(np.array_equal(e[0], r[0]) and np.array_equal(e[1], r[1])
-                           for e, r in zip(end_state, [ctx.state_f64(s) for s in sids]))
+    replay = snapshot(ctx, sids)
+    replay_identical = all(np.array_equal(e["pos"], r["pos"]) and np.array_equal(e["vel"], r["vel"])
+                           and np.array_equal(e["sed"], r["sed"]) for e, r in zip(end_state, replay))
 
+    counters = {}
+    if os.path.exists(COUNTERS):
+        with open(COUNTERS) as f:
+            counters = json.load(f).get(a.workload, {})
     value = N * a.steps / elapsed
     dom = max((k for k in kernels if kernels[k]["bytes"]), key=lambda k: kernels[k]["avg_ms"])
-    dk = kernels[dom]
-    achieved = dk["bytes"] / (dk["avg_ms"] * 1e-3) / 1e9
-    traffic = None
-    if os.path.exists(PROFILE_TRAFFIC):
-        with open(PROFILE_TRAFFIC) as f:
-            tr = json.load(f)
-        traffic = tr.get(a.workload, {}).get(dom)
-    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=traffic, kernel=dom,
-                    bytes_per_launch=dk["bytes"], avg_ms=dk["avg_ms"])
-
+    first = settle + a.warmup + 1
     out = dict(metric="particle-steps/s at <=1e-5 rel density err vs FP64; HBM GB/s fraction",
                value=value, unit="particle-steps/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
                ms_per_step=1e3 * elapsed / a.steps, higher_is_better=True, scaling="weak",
                vs_baseline=None, dtype="fp32", data="synthetic",
                config=dict(workload=f"{a.workload}: {desc}", particles=N, scenes=len(scs),
-                           precision="fp32 (compensated density sums, FP64-accumulated force sums)", block=blk, force_block=fblk, density_xcd_run=rm,
-                           x_slabs=ctx.get_subcells(), yz_split=ctx.get_yz_split()),
-               roofline=roofline)
+                           window=f"steps {first}..{first + a.steps - 1} of the {run_steps}-step run "
+                                  f"({settle} settle + {a.warmup} warm-up steps before it, untimed)",
+                           precision="fp32 (Kahan-compensated density and force sums)", block=blk,
+                           force_block=fblk, density_xcd_run=rm, x_slabs=ctx.get_subcells(),
+                           yz_split=ctx.get_yz_split()),
+               roofline=roofline(dom, kernels[dom], counters))
+    out["roofline_force"] = roofline("force", kernels["force"], counters)
     out["value_kernel_by_kernel"] = N * a.steps / prof_elapsed
     out["replay_bit_identical"] = replay_identical
     out["kernels_ms"] = {k: round(v["avg_ms"], 5) for k, v in kernels.items()}
-    out["mass_drift_rel"] = drift
+    out["terrain_contact"] = dict(particles_exchanging=int((sed1 != sed0).sum()),
+                                  particles_exchanging_frac=float((sed1 != sed0).mean()),
+                                  particles_carrying_sediment=int((sed1 > 0).sum()),
+                                  terrain_cells_changed=cells_changed)
+    out["mass_drift_rel_max_per_scene"] = float(drift.max())
     out["mass_units_exact"] = units_exact
-    out["nonfinite"] = status
+    out["status"] = status
 
-    if a.no_extras:
-        print(json.dumps(out), flush=True)
-        return
-    # neighbour statistics of the state the timed region ended in
-    counts = np.concatenate([ctx.debug_neighbours(s, csr=False)[0] for s in sids])
-    rho = np.concatenate([ctx.debug_density(s) for s in sids])
-    rho0 = np.array([s.params.rest_density for s in scs for _ in range(s.n)])
-    out["neighbours"] = dict(mean=float(counts.mean()), p50=float(np.median(counts)),
-                             p99=float(np.percentile(counts, 99)), max=int(counts.max()))
-    out["density_over_rest"] = dict(mean=float((rho / rho0).mean()), max=float((rho / rho0).max()))
-
-    log("bench: density error vs the FP64 path")
-    out["density_rel_err_max"] = density_error(ctx, sids, scs, blk, rm)
-    log("bench: accuracy protocol")
-    # after 50 steps as specified; after 400 the cfg2 column is on the
-    # terrain, so the height and sediment changes are compared too
-    out["accuracy"] = accuracy_protocol(scs, blk, rm, 50)
-    out["accuracy_400"] = accuracy_protocol(scs, blk, rm, 400)
+    if not a.no_extras:
+        counts = np.concatenate([ctx.debug_neighbours(s, csr=False)[0] for s in sids])
+        rho = np.concatenate([ctx.debug_density(s) for s in sids])
+        rho0 = np.concatenate([np.full(s.n, s.params.rest_density) for s in scs])
+        out["neighbours"] = dict(mean=float(counts.mean()), p50=float(np.median(counts)),
+                                 p99=float(np.percentile(counts, 99)), max=int(counts.max()))
+        out["density_over_rest"] = dict(mean=float((rho / rho0).mean()), max=float((rho / rho0).max()))
+        log("bench: density error vs the FP64 path")
+        out["density_rel_err_max"] = density_error(ctx, sids, scs, end_state, tuning)
+        log("bench: accuracy protocol")
+        out["accuracy"] = accuracy_protocol(scs, start_state, tuning)
 
     if not a.no_fp64:
-        log("bench: FP64 plain GPU path")
-        c64 = sphe.Context(sphe.FP64)
-        for s in scs:
-            c64.add_synthetic(s)
-        c64.set_tuning(blk, rm)
-        c64.set_force_block(fblk)
+        log("bench: FP64 plain GPU path over the same window")
+        c64, s64 = context_from(scs, start_state, sphe.FP64, tuning)
         c64.step(a.warmup)
         c64.sync()
         t = time.perf_counter()
@@ -322,10 +403,13 @@ def main():
         out["speedup_vs_fp64"] = value / v64
         c64.close()
 
-    if not a.no_cpu and len(scs) == 1:
+    if not a.no_cpu:
         log("bench: CPU baseline (reference step on host cores)")
-        out["cpu_baseline"] = cpu_baseline(scs[0], a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(scs, a.cpu_seconds)
     print(json.dumps(out), flush=True)
+    if status:
+        log(f"bench: the step reported status {status} (1: non-finite state, 2: a particle it could not place)")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -201,7 +201,7 @@ int sphe_terrain_heights(sphe_ctx* ctx, int scene, float* out, size_t count);
 /* Fixed-point cell volumes, row = z. */
 int sphe_terrain_volume_fx(sphe_ctx* ctx, int scene, int64_t* out, size_t count);
 /* Restores fixed-point cell volumes read by sphe_terrain_volume_fx (count =
- * nx * nz, |v| <= 2^56, the scene's fixed exponent unchanged; erosion is
+ * nx * nz, |v| <= 2^62, the scene's fixed exponent unchanged; erosion is
  * not capped at a cell's volume, so a cell may be below the terrain base): with
  * sphe_set_particles_f64 (sediment included) a checkpoint of a scene, from
  * which the steps continue bit for bit.  No counterpart in the reference,
@@ -244,6 +244,12 @@ int sphe_use_graph(sphe_ctx* ctx, int enable);
  * terrain volume, carried sediment, and the exact mod-2^64 sum of all
  * fixed-point volume units (conserved exactly by every step). */
 int sphe_mass(sphe_ctx* ctx, int scene, double* terrain, double* sediment, uint64_t* units);
+/* The same for every scene of the context in one launch (per scene a fixed
+ * number of blocks whose partials are added in block order, so the sums are
+ * deterministic): terrain[s], sediment[s], units[s] for each scene s; the
+ * arrays hold count >= the scene count entries (NULL: not wanted).  No
+ * counterpart in the reference. */
+int sphe_mass_all(sphe_ctx* ctx, double* terrain, double* sediment, uint64_t* units, int count);
 
 /* Non-finite state seen by the last step (positions or velocities); 0 if all
  * finite.  The reference produces NaN velocity for a particle at rest

@@ -1818,6 +1818,74 @@ __global__ __launch_bounds__(256) void k_mass_partial(DevState D, int scene, dou
     }
 }
 
+// The same bookkeeping for every scene in one launch: block (b, s) takes
+// cells and particles b, b + SPHE_MASS_BLOCKS_PER_SCENE blocks ... of scene
+// s and writes its partial to part[3 (s B + b) ..]; k_mass_scenes_final adds
+// a scene's B partials in block order.  Deterministic for a given layout.
+// Every partial and result is written by each launch: the buffers need no
+// clearing between calls.
+__global__ __launch_bounds__(256) void k_mass_scenes(DevState D, double* __restrict__ part)
+{
+    __shared__ double s_t[256];
+    __shared__ double s_s[256];
+    __shared__ unsigned long long s_u[256];
+    const int scene = (int)blockIdx.y;
+    const SceneDev& S = D.scenes[scene];
+    const int64_t ncells = (int64_t)S.tnx * S.tnz;
+    const double unit = ldexp(1.0, -S.fx_exp);
+    double st = 0.0, ss = 0.0;
+    unsigned long long su = 0ull;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t t = first; t < ncells; t += stride) {
+        const int64_t v = D.vol[S.t_off + t];
+        st += (double)v * unit;
+        su += (unsigned long long)v;
+    }
+    for (int64_t p = first; p < S.n; p += stride) {
+        const int64_t v = D.u_sed[S.p_begin + p];
+        ss += (double)v * unit;
+        su += (unsigned long long)v;
+    }
+    s_t[threadIdx.x] = st;
+    s_s[threadIdx.x] = ss;
+    s_u[threadIdx.x] = su;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s_t[threadIdx.x] += s_t[threadIdx.x + o];
+            s_s[threadIdx.x] += s_s[threadIdx.x + o];
+            s_u[threadIdx.x] += s_u[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* q = part + 3 * ((int64_t)scene * gridDim.x + blockIdx.x);
+        q[0] = s_t[0];
+        q[1] = s_s[0];
+        q[2] = __longlong_as_double((long long)s_u[0]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mass_scenes_final(const double* __restrict__ part, int nscenes, int nblocks,
+                                                           double* __restrict__ out)
+{
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= nscenes) return;
+    double st = 0.0, ss = 0.0;
+    unsigned long long su = 0ull;
+    const double* q = part + 3 * (int64_t)s * nblocks;
+    for (int b = 0; b < nblocks; ++b) {
+        st += q[3 * b];
+        ss += q[3 * b + 1];
+        su += (unsigned long long)__double_as_longlong(q[3 * b + 2]);
+    }
+    // planar: terrain[nscenes], sediment[nscenes], units[nscenes]
+    out[s] = st;
+    out[nscenes + s] = ss;
+    out[2 * nscenes + s] = __longlong_as_double((long long)su);
+}
+
 __global__ void k_mass_final(const double* __restrict__ part, int nblocks, double* __restrict__ out)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -2065,6 +2133,13 @@ void sphe_launch_mass(const DevState& D, int scene, double* part, double* out, h
 {
     k_mass_partial<<<sphe_mass_blocks(), 256, 0, st>>>(D, scene, part);
     k_mass_final<<<1, 64, 0, st>>>(part, sphe_mass_blocks(), out);
+}
+
+void sphe_launch_mass_scenes(const DevState& D, double* part, double* out, hipStream_t st)
+{
+    if (D.nscenes == 0) return;
+    k_mass_scenes<<<dim3(SPHE_MASS_BLOCKS_PER_SCENE, (unsigned)D.nscenes), 256, 0, st>>>(D, part);
+    k_mass_scenes_final<<<grid_of(D.nscenes, 256), 256, 0, st>>>(part, D.nscenes, SPHE_MASS_BLOCKS_PER_SCENE, out);
 }
 
 // ------------------------------------------------------------ calibration

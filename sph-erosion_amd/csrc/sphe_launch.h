@@ -42,4 +42,9 @@ void sphe_launch_neighbours(const DevState& D, int scene, const int64_t* off, in
 void sphe_launch_keys(const DevState& D, int scene, int32_t* keys, hipStream_t st);
 int sphe_mass_blocks();
 void sphe_launch_mass(const DevState& D, int scene, double* part, double* out, hipStream_t st);
+// Blocks per scene of the all-scene bookkeeping: part holds
+// 3 * SPHE_MASS_BLOCKS_PER_SCENE * nscenes doubles, out 3 * nscenes (terrain
+// volumes of all scenes, then their sediment, then their unit sums).
+#define SPHE_MASS_BLOCKS_PER_SCENE 32
+void sphe_launch_mass_scenes(const DevState& D, double* part, double* out, hipStream_t st);
 void sphe_launch_copy(const void* a, void* b, int64_t n16, hipStream_t st);

@@ -75,6 +75,11 @@ struct sphe_ctx {
     bool counted = false;       // key / arank / cnt prepared for the next scan
     bool heights_stale = false; // the last step eroded; heights follow in the next step's scan
     bool stepped = false;       // sorted buffers hold the last step's pre-step state
+    // state epoch: bumped by every call that changes the device state or how
+    // it is read; the verification hooks' sort of the current state is kept
+    // while its epoch is current
+    uint64_t epoch = 1;
+    uint64_t sorted_epoch = 0;
 
     DevState D{};
     std::vector<void*> allocs;
@@ -107,6 +112,9 @@ struct sphe_ctx {
 
     double* d_mass_part = nullptr;
     double* d_mass_out = nullptr;
+    // sphe_mass_all: per-scene block partials, then the per-scene results
+    double* d_mass_all = nullptr;
+    size_t mass_all_cap = 0;  // doubles
     double* d_inspect = nullptr;
     // id -> sorted slot map of the last step (sphe_get_particle), rebuilt on
     // the first inspection after a step; d_slot: the slot of a searched id
@@ -388,6 +396,7 @@ void free_device(sphe_ctx* c)
     c->stepped = false;
     c->table_uploaded = false;
     c->heights_stale = false;
+    ++c->epoch;
     c->d_idmap = nullptr;
     c->idmap_len = 0;
     c->idmap_valid = false;
@@ -609,6 +618,7 @@ int upload(sphe_ctx* c)
     c->device_valid = true;
     c->counted = false;
     c->stepped = false;
+    ++c->epoch;
     return SPHE_OK;
 }
 
@@ -663,6 +673,7 @@ int begin_host_edit(sphe_ctx* c)
     c->device_valid = false;
     c->stepped = false;
     c->layout_valid = false;
+    ++c->epoch;
     return SPHE_OK;
 }
 
@@ -705,6 +716,7 @@ int upload_table(sphe_ctx* c, bool force)
         HIPCHK(c, hipMemcpyAsync((void*)c->D.scenes, c->table.data(), t.size() * sizeof(SceneDev),
                                  hipMemcpyHostToDevice, c->stream));
     c->table_uploaded = true;
+    ++c->epoch;  // new constants: a kept sort of the state no longer holds
     return SPHE_OK;
 }
 
@@ -906,6 +918,7 @@ int resize_lists(sphe_ctx* c, int maxn)
     c->D.pool_cap = pool;
     c->D.maxn = maxn;
     c->maxn = maxn;
+    ++c->epoch;  // the new lists hold nothing yet
     return SPHE_OK;
 }
 
@@ -972,6 +985,7 @@ int do_steps(sphe_ctx* c, int nsteps)
     c->step_sub_log2 = c->D.sub_log2;
     c->idmap_valid = false;
     c->host_valid = false;
+    ++c->epoch;
     return SPHE_OK;
 }
 
@@ -992,12 +1006,15 @@ int collect_profile(sphe_ctx* c)
     return SPHE_OK;
 }
 
-// Sorts the current state without stepping (verification hooks).
+// Sorts the current state and runs the density pass without stepping
+// (verification hooks).  Kept until the state epoch changes, so the hooks of
+// several scenes share one pass.
 int sort_current(sphe_ctx* c)
 {
     int r = prepare(c);
     if (r) return r;
     if (c->D.P == 0) return SPHE_OK;
+    if (c->sorted_epoch == c->epoch && !c->stepped) return SPHE_OK;
     if (!c->counted) {
         r = recount(c);
         if (r) return r;
@@ -1009,6 +1026,7 @@ int sort_current(sphe_ctx* c)
     c->counted = false;
     c->stepped = false;
     c->idmap_valid = false;
+    c->sorted_epoch = c->epoch;
     // the sorted buffers now hold the current state; keep the unsorted
     // state authoritative for the next step (its counts were consumed)
     return SPHE_OK;
@@ -1031,7 +1049,10 @@ const Scene* scene_of(const sphe_ctx* c, int scene)
     if (!var) return fail((ctx), SPHE_ERANGE, "scene %d out of range", (int)(idx))
 
 // Fixed-point unit 2^-fx_exp of the scene's volumes: chosen so that twice the
-// tallest column the box allows stays below 2^56 units.
+// tallest column the box allows stays below 2^50 units.  That leaves a factor
+// of 2^12 below the largest volume a cell may hold (2^62): sediment carried
+// from many cells can be deposited into one (a corner the water piles into),
+// so a cell's volume is not bounded by its own column.
 int set_terrain_heights(sphe_ctx* c, Scene& s, const float* heights)
 {
     const size_t nt = (size_t)s.tnx * s.tnz;
@@ -1040,7 +1061,7 @@ int set_terrain_heights(sphe_ctx* c, Scene& s, const float* heights)
         for (size_t t = 0; t < nt; ++t) top = std::max(top, (double)heights[t]);
     const double col = std::max(1e-30, top - (double)s.t_org[1] + (double)s.t_cell);
     const double vmax = col * 2.0 * (double)s.t_cell * (double)s.t_cell;
-    s.fx_exp = std::max(-60, std::min(100, 56 - (int)std::ceil(std::log2(vmax))));
+    s.fx_exp = std::max(-60, std::min(100, 50 - (int)std::ceil(std::log2(vmax))));
     s.vol.assign(nt, 0);
     const double area = (double)s.t_cell * (double)s.t_cell;
     const double up = std::ldexp(1.0, s.fx_exp);
@@ -1124,6 +1145,7 @@ int sphe_destroy(sphe_ctx* c)
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     free_device(c);
     if (c->d_mesh) (void)hipFree(c->d_mesh);
+    if (c->d_mass_all) (void)hipFree(c->d_mass_all);
     if (c->h_ovf) (void)hipHostFree(c->h_ovf);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1561,7 +1583,7 @@ int sphe_set_terrain_volume_fx(sphe_ctx* c, int scene, const int64_t* vol, size_
     // erosion is not capped at a cell's volume, so a cell can go below the
     // terrain base (a negative volume): any value in the fixed-point range
     for (size_t t = 0; t < nt; ++t)
-        if (vol[t] < -((int64_t)1 << 56) || vol[t] > ((int64_t)1 << 56))
+        if (vol[t] < -((int64_t)1 << 62) || vol[t] > ((int64_t)1 << 62))
             return fail(c, SPHE_ERANGE, "cell %zu volume out of range", t);
     int r = begin_host_edit(c);
     if (r) return r;
@@ -1731,6 +1753,35 @@ int sphe_mass(sphe_ctx* c, int scene, double* terrain, double* sediment, uint64_
     return SPHE_OK;
 }
 
+int sphe_mass_all(sphe_ctx* c, double* terrain, double* sediment, uint64_t* units, int count)
+{
+    if (!c) return SPHE_EARG;
+    const int ns = (int)c->scenes.size();
+    if (count < ns) return fail(c, SPHE_EARG, "outputs hold %d scenes, the context has %d", count, ns);
+    if (ns > 65535) return fail(c, SPHE_ERANGE, "more than 65535 scenes");
+    if (ns == 0) return SPHE_OK;
+    int r = prepare(c);
+    if (r) return r;
+    const size_t part = (size_t)3 * SPHE_MASS_BLOCKS_PER_SCENE * ns, outn = (size_t)3 * ns;
+    if (c->mass_all_cap < part + outn) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->d_mass_all) (void)hipFree(c->d_mass_all);
+        c->d_mass_all = nullptr;
+        c->mass_all_cap = 0;
+        HIPCHK(c, try_malloc(&c->d_mass_all, (part + outn) * 8));
+        c->mass_all_cap = part + outn;
+    }
+    double* out = c->d_mass_all + part;
+    sphe_launch_mass_scenes(c->D, c->d_mass_all, out, c->stream);
+    HIPCHK(c, hipGetLastError());
+    // planar results: terrain volumes, sediment, unit sums of all scenes
+    if (terrain) HIPCHK(c, hipMemcpyAsync(terrain, out, (size_t)ns * 8, hipMemcpyDeviceToHost, c->stream));
+    if (sediment) HIPCHK(c, hipMemcpyAsync(sediment, out + ns, (size_t)ns * 8, hipMemcpyDeviceToHost, c->stream));
+    if (units) HIPCHK(c, hipMemcpyAsync(units, out + 2 * ns, (size_t)ns * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SPHE_OK;
+}
+
 int sphe_status(sphe_ctx* c, uint32_t* nonfinite)
 {
     if (!c || !nonfinite) return SPHE_EARG;
@@ -1846,6 +1897,7 @@ int sphe_set_lanes(sphe_ctx* c, int lanes)
 {
     if (!c) return SPHE_EARG;
     if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) return fail(c, SPHE_EARG, "lanes must be 1, 2, 4 or 8");
+    if (lanes != c->lanes) ++c->epoch;  // a kept density pass used the old split
     c->lanes = lanes;
     return SPHE_OK;
 }
@@ -1890,6 +1942,7 @@ int sphe_set_subcells(sphe_ctx* c, int slabs)
         while (sl > 0 && (c->D.C << sl) > INT32_MAX) --sl;
         if (sl != c->D.sub_log2) {
             c->D.sub_log2 = sl;
+            ++c->epoch;
             c->counted = false;  // the keys of the counted state carry the slab
         }
     }
@@ -1966,6 +2019,7 @@ int sphe_restore_state(sphe_ctx* c)
     }
     c->host_valid = true;
     c->device_valid = false;  // uploaded (and counted) on the next use
+    ++c->epoch;
     c->stepped = false;
     return SPHE_OK;
 }

@@ -253,6 +253,17 @@ class Context:
         self._chk(lib().sphe_mass(self._p, scene, C.byref(t), C.byref(s), C.byref(u)), "mass")
         return t.value, s.value, u.value
 
+    def mass_all(self):
+        """(terrain, sediment, units) arrays over every scene, one launch."""
+        n = lib().sphe_scene_count
+        cnt = C.c_int()
+        self._chk(n(self._p, C.byref(cnt)), "scene_count")
+        t = np.zeros(cnt.value)
+        s = np.zeros(cnt.value)
+        u = np.zeros(cnt.value, dtype=np.uint64)
+        self._chk(lib().sphe_mass_all(self._p, _ptr(t), _ptr(s), _ptr(u), cnt.value), "mass_all")
+        return t, s, u
+
     def status(self):
         v = C.c_uint32()
         self._chk(lib().sphe_status(self._p, C.byref(v)), "status")

@@ -353,3 +353,21 @@ def test_inspection_keeps_the_steps_order_after_tuning_changes():
         rec = ctx.particle(sid, pid)
         assert f32_bits_equal(rec.Acceleration[:], tw["acc"][pid])
         assert f32_bits_equal(rec.PressureForce[:], tw["pforce"][pid])
+
+
+def test_mass_all_equals_per_scene_mass():
+    """sphe_mass_all (every scene in one launch) gives each scene's terrain
+    and sediment totals and unit sums; the unit sums equal the per-scene
+    reductions exactly and the FP64 totals within rounding of their order."""
+    scs = [scenes.cfg1(seed_idx=k, n=1000 + 900 * k) for k in range(6)]
+    ctx = sphe.Context()
+    sids = [ctx.add_synthetic(s) for s in scs]
+    ctx.step(20)
+    t, s, u = ctx.mass_all()
+    for k, sid in enumerate(sids):
+        t1, s1, u1 = ctx.mass(sid)
+        assert u[k] == u1
+        assert abs(t[k] - t1) <= 1e-12 * t1 and abs(s[k] - s1) <= 1e-12 * max(s1, 1e-30)
+    assert s.sum() > 0
+    t2, s2, u2 = ctx.mass_all()
+    assert np.array_equal(t, t2) and np.array_equal(s, s2) and np.array_equal(u, u2)

@@ -512,7 +512,7 @@ def test_checkpoint_continues_bit_for_bit():
     with pytest.raises(Exception):
         b.set_terrain_volume_fx(sb, vol.ravel()[:-1])
     with pytest.raises(Exception):
-        b.set_terrain_volume_fx(sb, np.full(vol.size, -(2 ** 57), np.int64))  # outside |v| <= 2^56
+        b.set_terrain_volume_fx(sb, np.full(vol.size, -(2 ** 62) - 1, np.int64))  # outside |v| <= 2^62
 
 
 def test_full_size_column_properties():

@@ -246,6 +246,46 @@ def test_deposition_and_band():
 
 
 @pytest.mark.parametrize("real", [4, 8])
+def test_erosion_below_the_terrain_base(real):
+    """Erosion is not capped at a cell's volume: a fast particle over a flat
+    terrain at its base (volume 0) takes volume from the cell, which goes
+    negative (its height below the base) while water and sediment units stay
+    exactly conserved."""
+    T, fx = 8, 60
+    vol = np.zeros((T, T), np.int64)
+    sc = terrain_scene(T, fx)
+    c = O.derive(sc)
+    hts = O.heights(c, vol)
+    pos = np.array([[0.3, 0.02, 0.7]])  # inside the band above height 0
+    vel = np.array([[0.6, 0.0, 0.8]])
+    st = dict(pos=pos, vel=vel, sed=np.zeros(1, np.int64), vol=vol, height=hts)
+    for _ in range(2):
+        st = O.orc_step(sc, st["pos"], st["vel"], st["sed"], np.zeros(1, np.int32), st["vol"], st["height"],
+                        real=real, comp=1)
+        assert int(st["vol"].sum()) + int(st["sed"].sum()) == 0
+    k, i = int(0.7 * T), int(0.3 * T)
+    assert st["vol"][k, i] < 0 and st["height"].reshape(T, T)[k, i] < 0.0
+    assert st["sed"][0] == -st["vol"][k, i]
+
+
+@pytest.mark.parametrize("real", [4, 8])
+def test_erosion_exchange_saturates(real):
+    """One exchange is bounded by 2^52 fixed-point units: with the unit 2^-100
+    the capacity of a particle at 1 m/s (ke kc m / rho0, about 3e-7) is far
+    above 2^-48, and the particle takes exactly 2^52 units (an exchange above
+    the bound is clamped to it, not dropped)."""
+    T, fx = 8, 100
+    vol = np.zeros((T, T), np.int64)
+    sc = terrain_scene(T, fx)
+    hts = O.heights(O.derive(sc), vol)
+    pos = np.array([[0.3, 0.02, 0.7]])
+    vel = np.array([[0.6, 0.0, 0.8]])
+    out = O.orc_step(sc, pos, vel, np.zeros(1, np.int64), np.zeros(1, np.int32), vol, hts, real=real, comp=1)
+    assert out["sed"][0] == 2 ** 52
+    assert int(out["vol"].sum()) == -(2 ** 52)
+
+
+@pytest.mark.parametrize("real", [4, 8])
 def test_multi_step_mass_exact(real):
     T, fx = 16, 62
     rng = np.random.default_rng(2)
