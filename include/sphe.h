@@ -250,6 +250,17 @@ int sphe_mass(sphe_ctx* ctx, int scene, double* terrain, double* sediment, uint6
  * arrays hold count >= the scene count entries (NULL: not wanted).  No
  * counterpart in the reference. */
 int sphe_mass_all(sphe_ctx* ctx, double* terrain, double* sediment, uint64_t* units, int count);
+/* sphe_mass_all without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Its partial sums and
+ * results live in a static scratch array of the library (3.2 MiB), one copy
+ * per GPU, held until the process ends.  Limits: at most 4096 scenes
+ * (SPHE_ERANGE beyond); the context must already hold its device state (a
+ * step or sphe_mass_all has run since the last edit; SPHE_ESTATE otherwise),
+ * since building it allocates.  All calls on one GPU share the scratch:
+ * calls that overlap on one GPU (several contexts on different threads or
+ * streams) race and must be serialised by the caller. */
+int sphe_mass_all_noalloc(sphe_ctx* ctx, double* terrain, double* sediment, uint64_t* units, int count);
 
 /* Non-finite state seen by the last step (positions or velocities); 0 if all
  * finite.  The reference produces NaN velocity for a particle at rest

@@ -25,6 +25,9 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+
 #include "sphe_device.h"
 #include "sphe_launch.h"
 
@@ -2128,6 +2131,36 @@ void sphe_launch_keys(const DevState& D, int scene, int32_t* keys, hipStream_t s
 }
 
 int sphe_mass_blocks() { return 256; }
+
+// Scratch of sphe_mass_all_noalloc, for callers that cannot allocate device
+// memory: the per-scene block partials and the per-scene results of the
+// all-scene bookkeeping for up to SPHE_NOALLOC_MAX_SCENES scenes.  Every GPU
+// the library runs on holds its own copy until the process ends.  Host code
+// uses the address sphe_noalloc_scratch returns, never this name.
+static __device__ __align__(256) double scratch[(size_t)3 * (SPHE_MASS_BLOCKS_PER_SCENE + 1) *
+                                                SPHE_NOALLOC_MAX_SCENES];
+
+int sphe_noalloc_scratch(int device, double** ptr, size_t* doubles)
+{
+    static std::mutex mu;
+    static std::map<int, double*> addr;  // device -> address of its scratch
+    std::lock_guard<std::mutex> g(mu);
+    auto it = addr.find(device);
+    if (it == addr.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            *ptr = nullptr;
+            *doubles = 0;
+            return -1;
+        }
+        it = addr.emplace(device, (double*)p).first;
+    }
+    *ptr = it->second;
+    *doubles = sizeof(scratch) / sizeof(double);
+    return 0;
+}
 
 void sphe_launch_mass(const DevState& D, int scene, double* part, double* out, hipStream_t st)
 {

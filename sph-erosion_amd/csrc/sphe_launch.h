@@ -47,4 +47,12 @@ void sphe_launch_mass(const DevState& D, int scene, double* part, double* out, h
 // volumes of all scenes, then their sediment, then their unit sums).
 #define SPHE_MASS_BLOCKS_PER_SCENE 32
 void sphe_launch_mass_scenes(const DevState& D, double* part, double* out, hipStream_t st);
+// Most scenes sphe_mass_all_noalloc serves from the library's static scratch
+// (3 * (SPHE_MASS_BLOCKS_PER_SCENE + 1) doubles per scene, 3.2 MiB in all).
+#define SPHE_NOALLOC_MAX_SCENES 4096
+// Device address of that scratch on `device`, which must be the current
+// device: looked up with hipGetSymbolAddress on the first call for the
+// device and kept per device.  *doubles = its size.  0, or -1 when the
+// lookup failed.
+int sphe_noalloc_scratch(int device, double** ptr, size_t* doubles);
 void sphe_launch_copy(const void* a, void* b, int64_t n16, hipStream_t st);

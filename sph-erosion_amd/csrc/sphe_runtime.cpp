@@ -115,6 +115,10 @@ struct sphe_ctx {
     // sphe_mass_all: per-scene block partials, then the per-scene results
     double* d_mass_all = nullptr;
     size_t mass_all_cap = 0;  // doubles
+    // the library's static scratch on this context's device (the device
+    // address of its copy), for sphe_mass_all_noalloc; null if the lookup failed
+    double* noalloc_scratch = nullptr;
+    size_t noalloc_doubles = 0;
     double* d_inspect = nullptr;
     // id -> sorted slot map of the last step (sphe_get_particle), rebuilt on
     // the first inspection after a step; d_slot: the slot of a searched id
@@ -1130,6 +1134,9 @@ int sphe_create(int precision, sphe_ctx** out)
     }
     c->h_ovf = (uint32_t*)h;
     *(volatile uint32_t*)c->h_ovf = 0u;
+    // the static scratch's address on this (current) device; a failed lookup
+    // leaves only sphe_mass_all_noalloc unavailable, which then reports it
+    if (sphe_noalloc_scratch(dev, &c->noalloc_scratch, &c->noalloc_doubles) != 0) c->noalloc_scratch = nullptr;
     *out = c;
     return SPHE_OK;
 }
@@ -1775,6 +1782,32 @@ int sphe_mass_all(sphe_ctx* c, double* terrain, double* sediment, uint64_t* unit
     sphe_launch_mass_scenes(c->D, c->d_mass_all, out, c->stream);
     HIPCHK(c, hipGetLastError());
     // planar results: terrain volumes, sediment, unit sums of all scenes
+    if (terrain) HIPCHK(c, hipMemcpyAsync(terrain, out, (size_t)ns * 8, hipMemcpyDeviceToHost, c->stream));
+    if (sediment) HIPCHK(c, hipMemcpyAsync(sediment, out + ns, (size_t)ns * 8, hipMemcpyDeviceToHost, c->stream));
+    if (units) HIPCHK(c, hipMemcpyAsync(units, out + 2 * ns, (size_t)ns * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SPHE_OK;
+}
+
+int sphe_mass_all_noalloc(sphe_ctx* c, double* terrain, double* sediment, uint64_t* units, int count)
+{
+    if (!c) return SPHE_EARG;
+    const int ns = (int)c->scenes.size();
+    if (count < ns) return fail(c, SPHE_EARG, "outputs hold %d scenes, the context has %d", count, ns);
+    if (ns > SPHE_NOALLOC_MAX_SCENES)
+        return fail(c, SPHE_ERANGE, "sphe_mass_all_noalloc serves at most %d scenes, the context has %d",
+                    SPHE_NOALLOC_MAX_SCENES, ns);
+    if (!c->noalloc_scratch) return fail(c, SPHE_EHIP, "the library's static scratch is not available on this device");
+    if (ns == 0) return SPHE_OK;
+    // the layout and the device state are not built here (that allocates)
+    if (!c->layout_valid || !c->device_valid)
+        return fail(c, SPHE_ESTATE, "no device state to reduce: step the context (or call sphe_mass_all) first");
+    const size_t part = (size_t)3 * SPHE_MASS_BLOCKS_PER_SCENE * ns;
+    if (part + (size_t)3 * ns > c->noalloc_doubles) return fail(c, SPHE_ERANGE, "static scratch too small");
+    // every partial and result is rewritten by the launch: nothing to clear
+    double* out = c->noalloc_scratch + part;
+    sphe_launch_mass_scenes(c->D, c->noalloc_scratch, out, c->stream);
+    HIPCHK(c, hipGetLastError());
     if (terrain) HIPCHK(c, hipMemcpyAsync(terrain, out, (size_t)ns * 8, hipMemcpyDeviceToHost, c->stream));
     if (sediment) HIPCHK(c, hipMemcpyAsync(sediment, out + ns, (size_t)ns * 8, hipMemcpyDeviceToHost, c->stream));
     if (units) HIPCHK(c, hipMemcpyAsync(units, out + 2 * ns, (size_t)ns * 8, hipMemcpyDeviceToHost, c->stream));

@@ -90,6 +90,7 @@ SIGNATURES = {
     "sphe_use_graph": (I, [P, I]),
     "sphe_mass": (I, [P, I, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "sphe_mass_all": (I, [P, P, P, P, I]),
+    "sphe_mass_all_noalloc": (I, [P, P, P, P, I]),
     "sphe_status": (I, [P, C.POINTER(C.c_uint32)]),
     "sphe_profile": (I, [P, I]),
     "sphe_profile_read": (I, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(I)]),

@@ -264,6 +264,16 @@ class Context:
         self._chk(lib().sphe_mass_all(self._p, _ptr(t), _ptr(s), _ptr(u), cnt.value), "mass_all")
         return t, s, u
 
+    def mass_all_noalloc(self):
+        """mass_all from the library's static scratch (no device allocation)."""
+        cnt = C.c_int()
+        self._chk(lib().sphe_scene_count(self._p, C.byref(cnt)), "scene_count")
+        t = np.zeros(cnt.value)
+        s = np.zeros(cnt.value)
+        u = np.zeros(cnt.value, dtype=np.uint64)
+        self._chk(lib().sphe_mass_all_noalloc(self._p, _ptr(t), _ptr(s), _ptr(u), cnt.value), "mass_all_noalloc")
+        return t, s, u
+
     def status(self):
         v = C.c_uint32()
         self._chk(lib().sphe_status(self._p, C.byref(v)), "status")
