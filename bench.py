@@ -92,6 +92,30 @@ def snapshot(ctx, sids, f32=False):
     return out
 
 
+def save_states(ctx, sids, path, settled):
+    """Every scene's particles (FP32 values, id order) and terrain volumes,
+    concatenated, with the per-scene counts, in one .npz."""
+    st = snapshot(ctx, sids)
+    np.savez(path, settled=settled, n=np.array([len(s["ids"]) for s in st]),
+             pos=np.concatenate([s["pos"] for s in st]).astype(np.float32),
+             vel=np.concatenate([s["vel"] for s in st]).astype(np.float32),
+             ids=np.concatenate([s["ids"] for s in st]), sed=np.concatenate([s["sed"] for s in st]),
+             nvol=np.array([0 if s["vol"] is None else s["vol"].size for s in st]),
+             vol=np.concatenate([np.zeros(0, np.int64)] + [s["vol"].ravel() for s in st if s["vol"] is not None]))
+
+
+def load_states(ctx, sids, path):
+    with np.load(path) as d:  # each member read once (NpzFile reads on every access)
+        ns, nvs, pos, vel, ids, sed, vol = (d[k] for k in ("n", "nvol", "pos", "vel", "ids", "sed", "vol"))
+    p0 = v0 = 0
+    for s, n, nv in zip(sids, ns, nvs):
+        ctx.set_particles(s, pos[p0:p0 + n], vel=vel[p0:p0 + n], ids=ids[p0:p0 + n], sed=sed[p0:p0 + n])
+        if nv:
+            ctx.set_terrain_volume_fx(s, vol[v0:v0 + nv])
+        p0 += n
+        v0 += nv
+
+
 def context_from(scs, states, precision, tuning):
     import sphe
     c = sphe.Context(precision)
@@ -272,6 +296,13 @@ def main():
                     help="only the timed region (for profiler runs): no statistics, accuracy, FP64 or CPU legs")
     ap.add_argument("--mode", default="step", choices=["step", "copy"],
                     help="copy: time a plain 1 GiB device copy (HBM reference and counter calibration)")
+    ap.add_argument("--save-state", default=None,
+                    help="write the state the timed window starts from (after settle and warm-up) to this .npz")
+    ap.add_argument("--load-state", default=None,
+                    help="start from a state written by --save-state (no settle; give --warmup 0 for the same "
+                         "window): lets a profiler see only the window's kernels")
+    ap.add_argument("--block", type=int, default=None, help="density block size (with --no-tune)")
+    ap.add_argument("--force-block", type=int, default=None, help="force block size (with --no-tune)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -293,18 +324,37 @@ def main():
 
     ctx = sphe.Context(sphe.FP32)
     sids = [ctx.add_synthetic(s) for s in scs]
+    if a.load_state:
+        load_states(ctx, sids, a.load_state)
+        with np.load(a.load_state) as d:
+            settle = int(d["settled"])
+        # one step grows the neighbour lists to the state (as the settle steps
+        # did in the run that saved it); then back to the loaded state
+        ctx.save_state()
+        ctx.step(1)
+        ctx.sync()
+        ctx.restore_state()
+        log(f"bench: loaded the state after {settle} steps from {a.load_state}")
     if not a.no_tune:
         blk, rm = ctx.autotune(5)
         log(f"bench: autotuned block={blk} force_block={ctx.get_force_block()} xcd_remap={rm}")
+    elif a.block:
+        ctx.set_tuning(a.block, ctx.get_tuning()[1])
+    if a.no_tune and a.force_block:
+        ctx.set_force_block(a.force_block)
     blk, rm = ctx.get_tuning()
     fblk = ctx.get_force_block()
     tuning = (blk, rm, fblk)
-    for done in range(0, settle, 250):
-        ctx.step(min(250, settle - done))
-        ctx.sync()
-        log(f"bench: settled {min(done + 250, settle)} of {settle} steps")
+    if not a.load_state:
+        for done in range(0, settle, 250):
+            ctx.step(min(250, settle - done))
+            ctx.sync()
+            log(f"bench: settled {min(done + 250, settle)} of {settle} steps")
     ctx.step(a.warmup)
     ctx.sync()
+    if a.save_state:
+        save_states(ctx, sids, a.save_state, settle + a.warmup)
+        log(f"bench: saved the window's start state to {a.save_state}")
     ctx.save_state()
     start_state = snapshot(ctx, sids) if not a.no_extras else None
     sed0 = np.concatenate([ctx.sediment_fx(s) for s in sids])

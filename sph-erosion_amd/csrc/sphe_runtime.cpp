@@ -95,6 +95,11 @@ struct sphe_ctx {
     // end of every sequence and read by the host without a stream wait
     uint32_t* h_ovf = nullptr;
     uint32_t* h_ovf_dev = nullptr;
+    // completion of the last launches (one event per launch slot): before a
+    // launch the host waits for the one kInFlight launches back, so the
+    // overflow word it reads is at most that many launches old
+    hipEvent_t seq_ev[2] = {nullptr, nullptr};
+    uint64_t seq_count = 0;
 
     bool profiling = false;
     std::vector<ProfRec> pending;
@@ -950,6 +955,10 @@ int grow_lists(sphe_ctx* c)
 
 // Steps per captured graph launch of a long call.
 const int kSeqSteps = 32;
+// Launches queued ahead of the one whose published overflow word the host
+// reads (the size of sphe_ctx::seq_ev): the device never runs dry, and lists
+// too short for the state grow at most this many launches late.
+const int kInFlight = 2;
 
 int do_steps(sphe_ctx* c, int nsteps)
 {
@@ -968,6 +977,8 @@ int do_steps(sphe_ctx* c, int nsteps)
     // step at a time.  Before each launch the host looks at the overflow word
     // the last finished sequence published.
     for (int done = 0; done < nsteps;) {
+        hipEvent_t& ev = c->seq_ev[c->seq_count % kInFlight];
+        if (c->seq_count >= (uint64_t)kInFlight) HIPCHK(c, hipEventSynchronize(ev));
         r = grow_lists(c);
         if (r) return r;
         const int k = graph && nsteps - done >= kSeqSteps ? kSeqSteps : 1;
@@ -981,6 +992,8 @@ int do_steps(sphe_ctx* c, int nsteps)
             sphe_launch_publish(c->D, c->h_ovf_dev, c->stream);
         }
         HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(ev, c->stream));
+        ++c->seq_count;
         done += k;
         if (any_erosion(c, -1)) c->heights_stale = true;
         c->stepped = true;
@@ -1134,6 +1147,17 @@ int sphe_create(int precision, sphe_ctx** out)
     }
     c->h_ovf = (uint32_t*)h;
     *(volatile uint32_t*)c->h_ovf = 0u;
+    for (hipEvent_t& e : c->seq_ev) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            for (hipEvent_t& f : c->seq_ev)
+                if (f) (void)hipEventDestroy(f);
+            (void)hipHostFree(h);
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return SPHE_EHIP;
+        }
+    }
     // the static scratch's address on this (current) device; a failed lookup
     // leaves only sphe_mass_all_noalloc unavailable, which then reports it
     if (sphe_noalloc_scratch(dev, &c->noalloc_scratch, &c->noalloc_doubles) != 0) c->noalloc_scratch = nullptr;
@@ -1153,6 +1177,8 @@ int sphe_destroy(sphe_ctx* c)
     free_device(c);
     if (c->d_mesh) (void)hipFree(c->d_mesh);
     if (c->d_mass_all) (void)hipFree(c->d_mass_all);
+    for (hipEvent_t e : c->seq_ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->h_ovf) (void)hipHostFree(c->h_ovf);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
