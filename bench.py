@@ -280,6 +280,90 @@ def roofline(name, k, counters):
     return out
 
 
+def summarize_profiles(base, workload, steps):
+    """Summary of the rocprofv3 runs of one window (run from --load-state
+    with --no-extras): the kernel trace (prof_trace), one --pmc run each for
+    FETCH_SIZE, WRITE_SIZE, the SQ instruction counts (prof_SQ) and the stall
+    counters (prof_STALL), and FETCH_SIZE / WRITE_SIZE of the 1 GiB copy
+    (prof_copy_FETCH_SIZE, prof_copy_WRITE_SIZE), each written with -o run
+    under base/<name>.  The last `steps` launches of a kernel are the replay
+    (direct launches), the `steps` before them the window (graph launches).
+    Writes base/prof_summary.md and base/prof_counters.json (the per-kernel
+    HBM bytes and VALU instructions this bench reads from
+    profiles/counters.json)."""
+    import collections
+    import re
+    import sqlite3
+
+    def short(n):
+        m = re.search(r"(k_[a-z0-9_]+)", n)
+        return m.group(1) if m else n[:40]
+
+    def rows(name, sql):
+        db = os.path.join(base, name, "run_results.db")
+        if not os.path.exists(db):
+            return []
+        con = sqlite3.connect(db)
+        try:
+            return list(con.execute(sql))
+        finally:
+            con.close()
+
+    def counters(name):
+        out = collections.defaultdict(lambda: collections.defaultdict(list))
+        for kname, cname, val in rows(name, "select kernel_name, counter_name, value from counters_collection "
+                                            "order by dispatch_id"):
+            out[short(kname)][cname].append(val)
+        return out
+
+    def mean(v):
+        return sum(v) / len(v) if v else None
+
+    def win(v):  # the window's and the replay's launches (not the ones before)
+        return v[-2 * steps:]
+
+    durs = collections.defaultdict(list)
+    for name, dur in rows("prof_trace", "select name, duration from kernels order by start"):
+        durs[short(name)].append(dur / 1e3)  # us
+    total = sum(sum(v) for v in durs.values())
+    fetch, write = counters("prof_FETCH_SIZE"), counters("prof_WRITE_SIZE")
+    sq, stall = counters("prof_SQ"), counters("prof_STALL")
+    cf = mean(counters("prof_copy_FETCH_SIZE")["k_copy"]["FETCH_SIZE"])
+    cw = mean(counters("prof_copy_WRITE_SIZE")["k_copy"]["WRITE_SIZE"])
+    gib_kib = (1 << 30) / 1024
+    fscale, wscale = gib_kib / cf, gib_kib / cw
+    lines = [f"Copy calibration (1 GiB plain copy, `bench.py --mode copy`): FETCH_SIZE {cf:.0f} KiB per launch "
+             f"(scale {fscale:.3f}), WRITE_SIZE {cw:.0f} KiB (scale {wscale:.3f}).", "",
+             "| kernel | launches | avg us, window (graph) | avg us, replay (direct) | % of GPU time | "
+             "FETCH_SIZE KiB | WRITE_SIZE KiB | HBM traffic per launch (MB) | VALU instr per wave | waves |",
+             "|---|---|---|---|---|---|---|---|---|---|"]
+    out = {}
+    for k, v in sorted(durs.items(), key=lambda kv: -sum(kv[1])):
+        stepped = len(v) >= 2 * steps
+        wnd = mean(v[-2 * steps:-steps]) if stepped else mean(v)
+        rep = mean(v[-steps:]) if stepped else mean(v)
+        f, w = mean(win(fetch[k]["FETCH_SIZE"])), mean(win(write[k]["WRITE_SIZE"]))
+        vi, wv = mean(win(sq[k]["SQ_INSTS_VALU"])), mean(win(sq[k]["SQ_WAVES"]))
+        hbm = (f * fscale + w * wscale) * 1024 if f is not None and w is not None else None
+        lines.append(f"| {k} | {len(v)} | {wnd:.1f} | {rep:.1f} | {100 * sum(v) / total:.1f} | {f or 0:.0f} | "
+                     f"{w or 0:.0f} | {(hbm or 0) / 1e6:.1f} | {(vi / wv) if vi and wv else 0:.0f} | {wv or 0:.0f} |")
+        key = k.replace("k_", "").replace("_f32", "").replace("_count", "")
+        out[key] = dict(hbm_bytes=hbm, valu_insts=vi, waves=wv, avg_us_replay=rep, avg_us_window=wnd)
+        if k in stall:
+            out[key]["stall"] = {c: mean(win(x)) for c, x in stall[k].items()}
+    lines += ["", "Stall counters per launch (wave-cycles summed over the device; GRBM_GUI_ACTIVE summed over "
+                  "the XCDs):", ""]
+    for k in ("k_force_f32", "k_density", "k_reorder"):
+        for src in (stall, sq):
+            if k in src:
+                lines.append(f"- {k}: " + ", ".join(f"{c} {mean(win(x)):.3e}" for c, x in sorted(src[k].items())))
+    with open(os.path.join(base, "prof_summary.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(os.path.join(base, "prof_counters.json"), "w") as f:
+        json.dump({workload: out}, f, indent=1)
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,8 +378,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-extras", action="store_true",
                     help="only the timed region (for profiler runs): no statistics, accuracy, FP64 or CPU legs")
-    ap.add_argument("--mode", default="step", choices=["step", "copy"],
-                    help="copy: time a plain 1 GiB device copy (HBM reference and counter calibration)")
+    ap.add_argument("--mode", default="step", choices=["step", "copy", "summary"],
+                    help="copy: time a plain 1 GiB device copy (HBM reference and counter calibration); "
+                         "summary: summarise the rocprofv3 runs under --prof-dir (no GPU)")
+    ap.add_argument("--prof-dir", default="gpurun_out", help="where the rocprofv3 runs are (--mode summary)")
     ap.add_argument("--save-state", default=None,
                     help="write the state the timed window starts from (after settle and warm-up) to this .npz")
     ap.add_argument("--load-state", default=None,
@@ -304,6 +390,9 @@ def main():
     ap.add_argument("--block", type=int, default=None, help="density block size (with --no-tune)")
     ap.add_argument("--force-block", type=int, default=None, help="force block size (with --no-tune)")
     a = ap.parse_args()
+    if a.mode == "summary":
+        summarize_profiles(a.prof_dir, a.workload, a.steps)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -328,11 +417,13 @@ def main():
         load_states(ctx, sids, a.load_state)
         with np.load(a.load_state) as d:
             settle = int(d["settled"])
-        # one step grows the neighbour lists to the state (as the settle steps
-        # did in the run that saved it); then back to the loaded state
+        # the lists grow to the state as the settle steps grew them in the run
+        # that saved it: a step publishes the overflow, the next call grows
+        # the lists before its launch; then back to the loaded state
         ctx.save_state()
-        ctx.step(1)
-        ctx.sync()
+        for _ in range(2):
+            ctx.step(1)
+            ctx.sync()
         ctx.restore_state()
         log(f"bench: loaded the state after {settle} steps from {a.load_state}")
     if not a.no_tune:
@@ -427,6 +518,9 @@ def main():
     out["mass_drift_rel_max_per_scene"] = float(drift.max())
     out["mass_units_exact"] = units_exact
     out["status"] = status
+    # bit 4: some particle walked its stencil in the window (lists too short)
+    out["lists"] = dict(list_capacity=ctx.get_list_capacity(), overflow_chunk=ctx.get_overflow_capacity()[0],
+                        overflow_chunks=ctx.get_overflow_capacity()[1], stencil_walks_in_window=bool(status & 4))
 
     if not a.no_extras:
         counts = np.concatenate([ctx.debug_neighbours(s, csr=False)[0] for s in sids])
@@ -442,22 +536,31 @@ def main():
 
     if not a.no_fp64:
         log("bench: FP64 plain GPU path over the same window")
+        ctx.close()  # the FP64 context gets the device to itself
         c64, s64 = context_from(scs, start_state, sphe.FP64, tuning)
+        # one step sizes its neighbour lists to the state (as the settle did
+        # for the FP32 context), then back to the window's start
+        c64.save_state()
+        c64.step(1)
+        c64.sync()
+        c64.restore_state()
         c64.step(a.warmup)
         c64.sync()
         t = time.perf_counter()
         c64.step(a.steps)
         c64.sync()
         v64 = N * a.steps / (time.perf_counter() - t)
+        st64 = c64.status()
         out["fp64_value"] = v64
         out["speedup_vs_fp64"] = value / v64
+        out["fp64_stencil_walks_in_window"] = bool(st64 & 4)
         c64.close()
 
     if not a.no_cpu:
         log("bench: CPU baseline (reference step on host cores)")
         out["cpu_baseline"] = cpu_baseline(scs, a.cpu_seconds)
     print(json.dumps(out), flush=True)
-    if status:
+    if status & 3:
         log(f"bench: the step reported status {status} (1: non-finite state, 2: a particle it could not place)")
         sys.exit(3)
 

@@ -262,9 +262,15 @@ int sphe_mass_all(sphe_ctx* ctx, double* terrain, double* sediment, uint64_t* un
  * streams) race and must be serialised by the caller. */
 int sphe_mass_all_noalloc(sphe_ctx* ctx, double* terrain, double* sediment, uint64_t* units, int count);
 
-/* Non-finite state seen by the last step (positions or velocities); 0 if all
- * finite.  The reference produces NaN velocity for a particle at rest
- * outside the box (fluid_system.h:345); this reports it. */
+/* Status word of the last sphe_step call, 0 when all went well:
+ *   bit 1 (value 1): a non-finite position or velocity.  The reference
+ *     produces NaN velocity for a particle at rest outside the box
+ *     (fluid_system.h:345); this reports it.
+ *   bit 2 (value 2): a particle the sort could not place (inconsistent
+ *     counts); it was dropped, not written out of range.
+ *   bit 3 (value 4): a particle's neighbours fitted neither its list nor a
+ *     pool chunk, so the force pass walked its stencil again: the same
+ *     result, slower (the lists grow for the next steps). */
 int sphe_status(sphe_ctx* ctx, uint32_t* nonfinite);
 
 /* ---- measurement ------------------------------------------------------- */
@@ -319,12 +325,19 @@ int sphe_get_subcells(const sphe_ctx* ctx, int* slabs);
 int sphe_set_yz_split(sphe_ctx* ctx, int split);
 int sphe_get_yz_split(const sphe_ctx* ctx, int* split);
 /* Neighbour-list entries per particle (default 256; a multiple of 16).  A
- * particle with more neighbours continues its list in an overflow pool
- * (up to twice the capacity) or, without room there, walks its stencil in
- * the force pass; after any step that overflowed, the lists grow to 1.5x
- * the largest count.  The capacity never changes results. */
+ * particle with more neighbours continues its list in one chunk of an
+ * overflow pool or, without room there, walks its stencil in the force pass
+ * (status bit 4).  After a step sequence that ran out of room the lists grow
+ * to 1.5x the largest count, as far as all lists together fit in a quarter
+ * of the device's memory; longer neighbourhoods get pool chunks of 1.5x
+ * their excess over the list, and the pool 1.5x the chunks the last step
+ * claimed.  Setting the capacity sets the chunk size too.  Neither ever
+ * changes results. */
 int sphe_set_list_capacity(sphe_ctx* ctx, int entries);
 int sphe_get_list_capacity(const sphe_ctx* ctx, int* entries);
+/* Entries per overflow chunk and the pool's chunk count (0 before the
+ * context has a device layout). */
+int sphe_get_overflow_capacity(const sphe_ctx* ctx, int* chunk_entries, int64_t* chunks);
 int sphe_autotune(sphe_ctx* ctx, int steps_per_trial);
 /* Saves the particle and terrain state of every scene (a host copy) and
  * restores it later; stepping a restored state repeats the same steps bit

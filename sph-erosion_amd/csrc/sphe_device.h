@@ -106,13 +106,17 @@ struct DevState {
     uint32_t* nbl;
     uint32_t* nbc;
     int32_t maxn;
-    // overflow pool: a particle whose count passes maxn claims maxn more
-    // entries (pool[pbase[i] .. + maxn), neighbours maxn .. 2 maxn - 1) with
-    // an atomic on *ptop, which the scan of every step resets; pbase[i] is
-    // ~0u when it has none (the force pass then walks beyond maxn)
+    // overflow pool: a particle whose count passes maxn claims one chunk of
+    // ocap more entries (pool[pbase[i] .. + ocap), neighbours maxn ..
+    // maxn + ocap - 1) with an atomic on *ptop, which the scan of every step
+    // resets; pbase[i] is ~0u when it has none (the force pass then walks
+    // its stencil beyond maxn and sets status bit 4)
     uint32_t* pool;
     unsigned long long* ptop;
     uint32_t* pbase;
     int64_t pool_cap;
-    uint32_t* ovf;  // largest neighbour count that overflowed a list
+    int32_t ocap;
+    // ovf[0]: largest neighbour count that overflowed a list; ovf[1]:
+    // particles that found the pool full (both since the host last grew them)
+    uint32_t* ovf;
 };

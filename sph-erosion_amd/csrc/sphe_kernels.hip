@@ -873,11 +873,13 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
         // Writes list positions c0 .. c0+n-1 (c0 a multiple of 16, n <= 16)
         // from the ring: lane l copies positions c0 + l PER .. + PER-1, as
         // 16-byte (G = 8: 8-byte) stores of whole chunks.  The list (capacity
-        // cap, a multiple of 16) continues in the group's pool chunk of cap
-        // entries, claimed when the first chunk past cap is written (one
-        // 64-bit atomic; none left: the rest is not stored and the force
-        // pass walks the stencil); the list rows and pool chunks are 64-byte
-        // aligned.  Called by all lanes of the group.
+        // cap, a multiple of 16) continues in the group's pool chunk of ocap
+        // entries (a multiple of 16), claimed when the first chunk past cap is
+        // written (one 64-bit atomic; none left: the rest is not stored, the
+        // claim failure is counted and the force pass walks the stencil); the
+        // list rows and pool chunks are 64-byte aligned.  Called by all lanes
+        // of the group.
+        const uint32_t ocap = (uint32_t)D.ocap;
         auto flush = [&](uint32_t c0, uint32_t n) {
             constexpr int PER = 16 / G;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -891,12 +893,13 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
                     claimed = true;
                     uint32_t b0 = ~0u;
                     if (l == 0) {
-                        const unsigned long long t = atomicAdd(D.ptop, (unsigned long long)cap);
-                        b0 = t + cap <= (unsigned long long)D.pool_cap ? (uint32_t)t : ~0u;
+                        const unsigned long long t = atomicAdd(D.ptop, (unsigned long long)ocap);
+                        b0 = t + ocap <= (unsigned long long)D.pool_cap ? (uint32_t)t : ~0u;
+                        if (b0 == ~0u) atomicAdd(&D.ovf[1], 1u);
                     }
                     pb = __shfl(b0, gfirst, SPHE_WAVE);
                 }
-                if (pb != ~0u && c0 - cap < cap) dst = D.pool + pb + (c0 - cap);
+                if (pb != ~0u && c0 - cap < ocap) dst = D.pool + pb + (c0 - cap);
             }
             if (dst) {
                 const uint32_t* src = (const uint32_t*)((char*)rowtab + (ring_rel | ((c0 * 4u) & RMASK))) + l * PER;
@@ -994,10 +997,23 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
         if (cnt & 15u) flush(cnt & ~15u, cnt & 15u);  // the last, partial chunk
     }
     group_merge<G>(sum);
+    {
+        // the largest count past the list: one atomic per wave (the lanes
+        // still here; the loop over the overflowing groups is wave-uniform)
+        uint64_t over = __ballot(cnt > cap);
+        if (over) {
+            uint32_t mx = 0;
+            for (; over; over &= over - 1) {
+                const uint32_t c = (uint32_t)__shfl(cnt, __ffsll((unsigned long long)over) - 1, SPHE_WAVE);
+                mx = c > mx ? c : mx;
+            }
+            const uint64_t here = __ballot(1);
+            if ((int)(threadIdx.x & (SPHE_WAVE - 1)) == __ffsll((unsigned long long)here) - 1) atomicMax(D.ovf, mx);
+        }
+    }
     if (l != 0) return;
     D.nbc[slot] = cnt;
     D.pbase[slot] = pb;
-    if (cnt > cap) atomicMax(D.ovf, cnt);
     const T rho = sum.value() * K.mc6;
     const T p = K.k * (rho - K.rho0);
     ((T*)D.rho)[slot] = rho;
@@ -1056,6 +1072,18 @@ template <typename T> struct Terms {
     T lap, fp0, fp1, fp2, fv0, fv1, fv2, nn0, nn1, nn2;
 };
 
+// a + b, term by term (plain arithmetic)
+template <typename T>
+__device__ __forceinline__ Terms<T> terms_sum(const Terms<T>& a, const Terms<T>& b)
+{
+    Terms<T> t;
+    t.lap = a.lap + b.lap;
+    t.fp0 = a.fp0 + b.fp0; t.fp1 = a.fp1 + b.fp1; t.fp2 = a.fp2 + b.fp2;
+    t.fv0 = a.fv0 + b.fv0; t.fv1 = a.fv1 + b.fv1; t.fv2 = a.fv2 + b.fv2;
+    t.nn0 = a.nn0 + b.nn0; t.nn1 = a.nn1 + b.nn1; t.nn2 = a.nn2 + b.nn2;
+    return t;
+}
+
 template <typename T>
 __device__ __forceinline__ Terms<T> pair_terms(const SceneConsts<T>& K, T dx, T dy, T dz, T d2, T a_i, T a_j,
                                                T b_j, const V4<T>& vi, const V4<T>& vj)
@@ -1088,9 +1116,10 @@ __device__ __forceinline__ Terms<T> pair_terms(const SceneConsts<T>& K, T dx, T 
     return t;
 }
 
-// Force sums of one lane.  A lane's neighbours enter in consecutive pairs:
-// the two terms are added in plain arithmetic (first + second) and the pair
-// sum enters the lane's sum; an odd last neighbour enters alone.
+// Force sums of one lane.  A lane's neighbours enter in consecutive groups
+// of four, added in plain arithmetic as (a + b) + (c + d) before the group
+// enters the lane's sum; the last one to three enter as a pair (a + b) and
+// then a single.
 // finish() applies the constants and forms the acceleration
 // (fluid_system.h:151-154, 163, 177, 318-321).
 // The force sums' accumulator: plain, Kahan-compensated, or (FP32 terms)
@@ -1123,6 +1152,16 @@ struct ForceAcc {
         fp0.add(a.fp0 + b.fp0); fp1.add(a.fp1 + b.fp1); fp2.add(a.fp2 + b.fp2);
         fv0.add(a.fv0 + b.fv0); fv1.add(a.fv1 + b.fv1); fv2.add(a.fv2 + b.fv2);
         nn0.add(a.nn0 + b.nn0); nn1.add(a.nn1 + b.nn1); nn2.add(a.nn2 + b.nn2);
+    }
+
+    // four consecutive terms as two plain pair sums ab = a + b, cd = c + d:
+    // ab + cd enters the sums
+    __device__ __forceinline__ void add4(const Terms<T>& ab, const Terms<T>& cd)
+    {
+        lap.add(ab.lap + cd.lap);
+        fp0.add(ab.fp0 + cd.fp0); fp1.add(ab.fp1 + cd.fp1); fp2.add(ab.fp2 + cd.fp2);
+        fv0.add(ab.fv0 + cd.fv0); fv1.add(ab.fv1 + cd.fv1); fv2.add(ab.fv2 + cd.fv2);
+        nn0.add(ab.nn0 + cd.nn0); nn1.add(ab.nn1 + cd.nn1); nn2.add(ab.nn2 + cd.nn2);
     }
 
     template <int G>
@@ -1165,8 +1204,9 @@ __device__ __forceinline__ void force_share_walk(const DevState& D, const SceneD
     const V4<T>* __restrict__ pos = (const V4<T>*)D.s_pos;
     const V4<T>* __restrict__ posa = (const V4<T>*)D.posa;
     const V4<T>* __restrict__ vel = (const V4<T>*)D.s_vel;
-    Terms<T> pend;
-    bool has = false;  // pend holds the first of a pair
+    // the start of a group of four a, b, c, d: ab = a (+ b), c
+    Terms<T> ab, c;
+    int np = 0;  // terms of the group so far
     uint32_t m = 0;    // canonical index of the next accepted neighbour
     if (l == 0) ++count;  // the particle itself
     visit_neighbours<T>(D, S, K, pos, pi.x, pi.y, pi.z, [&](uint32_t j, T dx, T dy, T dz, T d2) {
@@ -1179,11 +1219,15 @@ __device__ __forceinline__ void force_share_walk(const DevState& D, const SceneD
         nbmax = idj > nbmax ? idj : nbmax;
         const V4<T> vj = at(vel, j);
         const Terms<T> t = pair_terms<T>(K, dx, dy, dz, d2, a_i, at(posa, j).w, vj.w, vi, vj);
-        if (has) acc.add2(pend, t);
-        else pend = t;
-        has = !has;
+        if (np == 0) ab = t;
+        else if (np == 1) ab = terms_sum(ab, t);
+        else if (np == 2) c = t;
+        else acc.add4(ab, terms_sum(c, t));
+        np = (np + 1) & 3;
     });
-    if (has) acc.add(pend);
+    // the lane's last one to three terms: a, or the pair a + b (then c)
+    if (np != 0) acc.add(ab);
+    if (np == 3) acc.add(c);
 }
 
 // Dominant-axis box response (fluid_system.h:342-347 with collisionS
@@ -1298,8 +1342,9 @@ __device__ __forceinline__ int64_t erode_exchange(const SceneDev& S, const Scene
 // Lane l's share of the force sums from list entries l, l+G, l+2G, ... of
 // a particle's neighbour sequence (fetch(k) = the byte offset of its k-th
 // neighbour's slot in the V4<T> vectors): the lanes
-// of a particle differ by one neighbour at most.  Two pairs per iteration
-// (eight gathers in flight), then one pair, then a single.
+// of a particle differ by one neighbour at most.  Four neighbours per
+// iteration (eight gathers in flight), summed (a + b) + (c + d) before they
+// enter the lane's sums; then one pair, then a single.
 template <typename T, int SM, int G, typename F>
 __device__ __forceinline__ void force_list(const DevState& D, const SceneConsts<T>& K, int l, uint32_t cnt,
                                           const V4<T>& pi, const V4<T>& vi, T a_i, ForceAcc<T, SM>& acc,
@@ -1319,22 +1364,24 @@ __device__ __forceinline__ void force_list(const DevState& D, const SceneConsts<
         }
         const V4<T> pa = at_byte(posa, ja), va = at_byte(vel, ja), pb = at_byte(posa, jb), vb = at_byte(vel, jb);
         const V4<T> pc = at_byte(posa, jc), vc = at_byte(vel, jc), pd = at_byte(posa, jd), vd = at_byte(vel, jd);
+        Terms<T> ab, cd;
         {
             const T ax = pi.x - pa.x, ay = pi.y - pa.y, az = pi.z - pa.z;
             const T a2 = (ax * ax + ay * ay) + az * az;
             const T bx = pi.x - pb.x, by = pi.y - pb.y, bz = pi.z - pb.z;
             const T b2 = (bx * bx + by * by) + bz * bz;
-            acc.add2(pair_terms<T>(K, ax, ay, az, a2, a_i, pa.w, va.w, vi, va),
-                     pair_terms<T>(K, bx, by, bz, b2, a_i, pb.w, vb.w, vi, vb));
+            ab = terms_sum(pair_terms<T>(K, ax, ay, az, a2, a_i, pa.w, va.w, vi, va),
+                           pair_terms<T>(K, bx, by, bz, b2, a_i, pb.w, vb.w, vi, vb));
         }
         {
             const T ax = pi.x - pc.x, ay = pi.y - pc.y, az = pi.z - pc.z;
             const T a2 = (ax * ax + ay * ay) + az * az;
             const T bx = pi.x - pd.x, by = pi.y - pd.y, bz = pi.z - pd.z;
             const T b2 = (bx * bx + by * by) + bz * bz;
-            acc.add2(pair_terms<T>(K, ax, ay, az, a2, a_i, pc.w, vc.w, vi, vc),
-                     pair_terms<T>(K, bx, by, bz, b2, a_i, pd.w, vd.w, vi, vd));
+            cd = terms_sum(pair_terms<T>(K, ax, ay, az, a2, a_i, pc.w, vc.w, vi, vc),
+                           pair_terms<T>(K, bx, by, bz, b2, a_i, pd.w, vd.w, vi, vd));
         }
+        acc.add4(ab, cd);
     }
     for (; k + G < cnt; k += 2 * G) {
         const uint32_t ja = fetch(k);
@@ -1366,7 +1413,7 @@ __device__ __forceinline__ void force_list(const DevState& D, const SceneConsts<
 template <typename T, int SM, int G>
 __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S, const SceneConsts<T>& K,
                                            int64_t slot, int l, const V4<T>& pi, const V4<T>& vi, T a_i,
-                                           ForceAcc<T, SM>& acc)
+                                           ForceAcc<T, SM>& acc, bool report_walk)
 {
     const uint32_t cnt = D.nbc[slot];
     const uint32_t cap = (uint32_t)D.maxn;
@@ -1376,12 +1423,14 @@ __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S
         return;
     }
     const uint32_t pb = D.pbase[slot];
-    if (pb != ~0u && cnt - cap <= cap) {
+    if (pb != ~0u && cnt - cap <= (uint32_t)D.ocap) {
         const uint32_t* __restrict__ pool = D.pool;
         force_list<T, SM, G>(D, K, l, cnt, pi, vi, a_i, acc,
                                [&](uint32_t k) { return k < cap ? nbl[k] : pool[pb + (k - cap)]; });
         return;
     }
+    // no room in list and pool: the stencil again (same order, same result)
+    if (report_walk && l == 0) atomicOr(D.status, 4u);
     int32_t nbmax = -1, count = 0;
     force_share_walk<T, SM, G>(D, S, K, slot, l, pi, vi, a_i, acc, nbmax, count);
 }
@@ -1420,7 +1469,7 @@ __device__ __forceinline__ void force_body(const DevState& D, int nblk, int rema
     const T a_i = ((const V4<T>*)D.posa)[slot].w;
     ForceAcc<T, SM> acc;
     if (l == 0) acc.self_term(K, vi.w);
-    force_share<T, SM, G>(D, S, K, slot, l, pi, vi, a_i, acc);
+    force_share<T, SM, G>(D, S, K, slot, l, pi, vi, a_i, acc, true);
     acc.template merge<G>();
     if (l != 0) return;
     ForceOut<T> o;
@@ -1675,7 +1724,7 @@ __global__ __launch_bounds__(256) void k_accel(DevState D, int scene, double* __
     const T a_i = ((const V4<T>*)D.posa)[slot].w;
     ForceAcc<T, SM> acc;
     if (l == 0) acc.self_term(K, vi.w);
-    force_share<T, SM, G>(D, S, K, slot, l, pi, vi, a_i, acc);
+    force_share<T, SM, G>(D, S, K, slot, l, pi, vi, a_i, acc, false);
     acc.template merge<G>();
     if (l != 0) return;
     ForceOut<T> o;
@@ -1770,12 +1819,20 @@ __global__ void k_clear_status(uint32_t* status)
     if (threadIdx.x == 0 && blockIdx.x == 0) *status = 0u;
 }
 
-// Copies the list-overflow word into host memory mapped for the device (a
-// vector store), where the host reads it between step sequences without
-// waiting for the stream.
-__global__ void k_publish(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst)
+// Copies the list-overflow words (largest overflowing count, failed pool
+// claims) and the pool entries the last density pass claimed into host memory
+// mapped for the device (vector stores), where the host reads them between
+// step sequences without waiting for the stream.
+__global__ void k_publish(const uint32_t* __restrict__ ovf, const unsigned long long* __restrict__ ptop,
+                          uint32_t* __restrict__ dst)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *(volatile uint32_t*)dst = *src;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const unsigned long long t = *ptop;
+    volatile uint32_t* d = dst;
+    d[0] = ovf[0];
+    d[1] = ovf[1];
+    d[2] = (uint32_t)t;
+    d[3] = (uint32_t)(t >> 32);
 }
 
 // Mass bookkeeping of one scene: FP64 sums of terrain volume and carried
@@ -2054,7 +2111,7 @@ void sphe_launch_clear_status(const DevState& D, hipStream_t st) { k_clear_statu
 
 void sphe_launch_publish(const DevState& D, uint32_t* host_word, hipStream_t st)
 {
-    k_publish<<<1, 64, 0, st>>>(D.ovf, host_word);
+    k_publish<<<1, 64, 0, st>>>(D.ovf, D.ptop, host_word);
 }
 
 void sphe_launch_terrain_mesh(const float* H, int nx, int nz, float cell, float ox, float oz, float* out,

@@ -25,7 +25,8 @@ void sphe_launch_force(const DevState& D, int block, int remap, int lanes, hipSt
 void sphe_launch_heights(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void sphe_launch_two_phase(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void sphe_launch_clear_status(const DevState& D, hipStream_t st);
-// Copies the list-overflow word (D.ovf) to a host word mapped for the device.
+// Copies D.ovf[0], D.ovf[1] and the 64-bit *D.ptop (low, high) to four host
+// words mapped for the device.
 void sphe_launch_publish(const DevState& D, uint32_t* host_word, hipStream_t st);
 void sphe_launch_terrain_mesh(const float* H, int nx, int nz, float cell, float ox, float oz, float* out,
                               hipStream_t st);

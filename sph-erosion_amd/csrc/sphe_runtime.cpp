@@ -111,6 +111,8 @@ struct sphe_ctx {
     int fblock = 0;  // block size of the force pass (0: the density pass's)
     int remap = 128;  // density: runs of 128 consecutive blocks per XCD (L2 locality); force: plain order
     int maxn = 256;  // neighbour-list capacity per particle (grown when a step overflows it)
+    int ocap = 256;  // entries per overflow-pool chunk
+    size_t mem_total = 0;  // the device's memory (bounds the neighbour lists)
     int lanes = 4;   // lanes per particle in the density and force passes
     int sub_log2 = 2;  // x slabs per cell: 2^sub_log2
     int yz = 2;        // cells are 1/yz of the x edge along y and z
@@ -189,13 +191,16 @@ hipError_t try_malloc(T** p, size_t bytes)
 
 size_t real_size(int precision) { return precision == SPHE_FP64 ? sizeof(double) : sizeof(float); }
 
-// Overflow pool entries for list capacity maxn: room for an eighth of the
-// particles to hold twice the capacity (at least 64 of them).
-int64_t pool_entries(int64_t P, int maxn)
+// Overflow pool entries for `chunks` chunks of ocap entries (pool offsets
+// are 32-bit).
+int64_t pool_entries(int64_t chunks, int ocap)
 {
-    const int64_t n = std::max<int64_t>(P / 8, 64) * maxn;
-    return std::min<int64_t>(n, (int64_t)INT32_MAX);
+    return std::min<int64_t>(chunks * ocap, (int64_t)INT32_MAX);
 }
+
+// Overflow chunks of a new layout: one for an eighth of the particles (at
+// least 64).
+int64_t pool_chunks_initial(int64_t P) { return std::max<int64_t>(P / 8, 64); }
 
 // ---------------------------------------------------------------- derive
 
@@ -429,6 +434,14 @@ int dalloc(sphe_ctx* c, void** p, size_t bytes)
         ptr = (decltype(ptr))p_;                                  \
     } while (0)
 
+// Zeroes the four published words (largest overflowing count, failed pool
+// claims, pool entries claimed) in the mapped host block.
+void clear_published(sphe_ctx* c)
+{
+    volatile uint32_t* h = c->h_ovf;
+    for (int k = 0; k < 4; ++k) h[k] = 0u;
+}
+
 int build_layout(sphe_ctx* c)
 {
     free_device(c);
@@ -513,7 +526,8 @@ int build_layout(sphe_ctx* c)
     D.maxn = c->maxn;
     DALLOC(D.nbl, Pn * (size_t)D.maxn * 4);
     DALLOC(D.nbc, Pn * 4);
-    D.pool_cap = pool_entries(P, D.maxn);
+    D.ocap = c->ocap;
+    D.pool_cap = pool_entries(pool_chunks_initial(P), D.ocap);
     DALLOC(D.pool, (size_t)D.pool_cap * 4);
     DALLOC(D.pbase, Pn * 4);
     DALLOC(D.ptop, 64);
@@ -554,7 +568,7 @@ int build_layout(sphe_ctx* c)
     HIPCHK(c, hipMemsetAsync(D.ptop, 0, 64, c->stream));
     if (TC > 0) HIPCHK(c, hipMemsetAsync(D.vol, 0, (size_t)TC * 8, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    *(volatile uint32_t*)c->h_ovf = 0u;  // no overflow published for the new lists
+    clear_published(c);  // no overflow published for the new lists
     c->layout_valid = true;
     c->device_valid = false;
     return SPHE_OK;
@@ -839,12 +853,13 @@ void launch_step(sphe_ctx* c)
 
 std::string plan_signature(const sphe_ctx* c)
 {
-    char b[256];
-    snprintf(b, sizeof(b), "%d/%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%p/%d/%d/%d", c->precision, c->block,
+    // everything the captured kernels take by value (DevState) or as launch shape
+    char b[320];
+    snprintf(b, sizeof(b), "%d/%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%p/%d/%d/%d/%d/%lld", c->precision, c->block,
              force_block(c), c->remap,
              c->lanes, (long long)c->D.P, (long long)c->D.C, (long long)c->D.TC, (int)any_erosion(c, -1),
              (int)any_erosion(c, SPHE_EROSION_TWO_PHASE), (const void*)c->D.u_pos, (const void*)c->D.nbl,
-             (const void*)c->D.pool, c->D.maxn, c->D.sub_log2, c->D.yz);
+             (const void*)c->D.pool, c->D.maxn, c->D.sub_log2, c->D.yz, c->D.ocap, (long long)c->D.pool_cap);
     return b;
 }
 
@@ -895,61 +910,98 @@ int recount(sphe_ctx* c)
     return SPHE_OK;
 }
 
-// The neighbour lists hold maxn entries per particle.  When a step found a
-// larger count (those particles walked their stencil again in the force
-// pass, with the same result), the lists grow for the following steps.
-// New neighbour lists and overflow pool for capacity maxn (the old ones are
-// freed).  On an allocation failure the old ones stay: a particle without
-// room walks its stencil in the force pass, with the same result.
-int resize_lists(sphe_ctx* c, int maxn)
+// Replaces the neighbour lists (maxn entries per particle) and the overflow
+// pool (chunks of ocap entries) where their sizes change.  The contents need
+// not survive: every density pass rewrites them.  The old buffer is freed
+// before the new one is taken, so the peak is the larger of the two; if the
+// new size cannot be had, the old size is taken again and nothing changes
+// (a particle without room then walks its stencil: the same result, slower).
+int resize_lists(sphe_ctx* c, int maxn, int ocap, int64_t chunks)
 {
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    void* p = nullptr;
-    void* q = nullptr;
-    const int64_t pool = pool_entries(c->D.P, maxn);
-    // on a failed allocation the old lists stay
-    if (try_malloc(&p, (size_t)c->D.P * (size_t)maxn * 4) != hipSuccess) return SPHE_OK;
-    if (try_malloc(&q, (size_t)pool * 4) != hipSuccess) {
-        (void)hipFree(p);
-        return SPHE_OK;
-    }
-    for (void*& a : c->allocs) {
-        if (a == (void*)c->D.nbl) {
-            (void)hipFree(a);
-            a = p;
-        } else if (a == (void*)c->D.pool) {
-            (void)hipFree(a);
-            a = q;
+    const int64_t P = c->D.P;
+    const int64_t pool = pool_entries(chunks, ocap);
+    auto replace = [&](void* old, size_t old_bytes, size_t new_bytes, void** out) -> bool {
+        auto it = std::find(c->allocs.begin(), c->allocs.end(), old);
+        (void)hipFree(old);
+        void* p = nullptr;
+        bool ok = true;
+        if (try_malloc(&p, new_bytes) != hipSuccess) {
+            ok = false;
+            if (try_malloc(&p, old_bytes) != hipSuccess) p = nullptr;
         }
+        if (it != c->allocs.end()) *it = p;
+        *out = p;
+        return ok;
+    };
+    bool ok = true;
+    if (maxn != c->D.maxn) {
+        void* p = nullptr;
+        ok = replace(c->D.nbl, (size_t)P * c->D.maxn * 4, (size_t)P * maxn * 4, &p);
+        if (!p) return fail(c, SPHE_ENOMEM, "neighbour lists lost: no device memory");
+        c->D.nbl = (uint32_t*)p;
+        if (ok) c->D.maxn = maxn;
     }
-    c->D.nbl = (uint32_t*)p;
-    c->D.pool = (uint32_t*)q;
-    c->D.pool_cap = pool;
-    c->D.maxn = maxn;
-    c->maxn = maxn;
+    if (ok && pool != c->D.pool_cap) {
+        void* q = nullptr;
+        ok = replace(c->D.pool, (size_t)c->D.pool_cap * 4, (size_t)pool * 4, &q);
+        if (!q) return fail(c, SPHE_ENOMEM, "overflow pool lost: no device memory");
+        c->D.pool = (uint32_t*)q;
+        if (ok) c->D.pool_cap = pool;
+    }
+    if (ok) c->D.ocap = ocap;
+    c->maxn = c->D.maxn;
+    c->ocap = c->D.ocap;
     ++c->epoch;  // the new lists hold nothing yet
     return SPHE_OK;
 }
 
-// Grows the lists when a step sequence published a count above the
-// capacity.  The published word is read without waiting for the stream: it
-// may lag by the sequences still in flight, which only delays the growth
-// (an overflowing particle meanwhile continues in the pool or walks its
-// stencil, with the same result).
+// Longest list a particle gets before its neighbours continue in a chunk of
+// the overflow pool: the lists of all particles may take a quarter of the
+// device's memory (a multiple of 128 entries, at least 256, at most 2^20).
+// An overflowing particle costs two same-address atomics per step (the
+// largest count and the chunk claim), so lists long enough for every
+// neighbourhood are the fast case wherever memory allows them.
+int list_max(const sphe_ctx* c)
+{
+    const int64_t per_entry = (c->D.P + 64) * 4;
+    int64_t m = (int64_t)(c->mem_total / 4) / per_entry / 128 * 128;
+    return (int)std::min<int64_t>(std::max<int64_t>(m, 256), 1 << 20);
+}
+
+// Grows the lists or the pool when the last step sequence published that a
+// particle had no room: a count above the list and its chunk, or a chunk
+// claim that found the pool full.  The published words are read without
+// waiting for the stream; they lag by the launches still in flight, which
+// only delays the growth.  Policy: lists up to 1.5x the largest count, at
+// most list_max entries; longer neighbourhoods get chunks of 1.5x the excess
+// over the list; the pool holds 1.5x the chunks the last step claimed.
 int grow_lists(sphe_ctx* c)
 {
-    const uint32_t seen = *(volatile uint32_t*)c->h_ovf;
-    if (seen <= (uint32_t)c->D.maxn) return SPHE_OK;
+    volatile uint32_t* h = c->h_ovf;
+    const uint32_t seen = h[0], failed = h[1];
+    if (seen <= (uint32_t)(c->D.maxn + c->D.ocap) && failed == 0) return SPHE_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    uint32_t need = 0;
-    HIPCHK(c, hipMemcpy(&need, c->D.ovf, 4, hipMemcpyDeviceToHost));
-    need = std::max(need, seen);
-    const int step = 128;
-    const int r = resize_lists(c, (int)(((int64_t)need * 3 / 2 + step - 1) / step * step));
+    uint32_t w[2] = {0, 0};
+    unsigned long long claimed = 0;
+    HIPCHK(c, hipMemcpy(w, c->D.ovf, 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(&claimed, c->D.ptop, 8, hipMemcpyDeviceToHost));
+    const int64_t need = std::max<int64_t>(w[0], seen);
+    const int64_t fails = std::max<int64_t>(w[1], failed);
+    auto up = [](int64_t x, int64_t m) { return (x + m - 1) / m * m; };
+    int maxn = c->D.maxn;
+    if (need > maxn) maxn = (int)std::max<int64_t>(maxn, std::min<int64_t>(up(need * 3 / 2, 128), list_max(c)));
+    int ocap = c->D.ocap;
+    if (need > maxn + ocap) ocap = (int)up((need - maxn) * 3 / 2, 128);
+    const int64_t claims = (int64_t)(claimed / (unsigned long long)c->D.ocap) + fails;
+    int64_t chunks = c->D.pool_cap / c->D.ocap;
+    if (fails > 0 || ocap != c->D.ocap) chunks = std::max<int64_t>(64, up(claims * 3 / 2, 64));
+    chunks = std::min<int64_t>(chunks, INT32_MAX / ocap);
+    const int r = resize_lists(c, maxn, ocap, chunks);
     if (r) return r;
-    HIPCHK(c, hipMemsetAsync(c->D.ovf, 0, 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->D.ovf, 0, 8, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    *(volatile uint32_t*)c->h_ovf = 0u;
+    clear_published(c);
     return SPHE_OK;
 }
 
@@ -1132,6 +1184,11 @@ int sphe_create(int precision, sphe_ctx** out)
     sphe_ctx* c = new sphe_ctx();
     c->precision = precision;
     c->device = dev;
+    size_t mem_free = 0;
+    if (hipMemGetInfo(&mem_free, &c->mem_total) != hipSuccess) {
+        (void)hipGetLastError();
+        c->mem_total = 0;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return SPHE_EHIP;
@@ -1146,7 +1203,7 @@ int sphe_create(int precision, sphe_ctx** out)
         return SPHE_EHIP;
     }
     c->h_ovf = (uint32_t*)h;
-    *(volatile uint32_t*)c->h_ovf = 0u;
+    clear_published(c);  // pinned memory may come back holding an earlier owner's words
     for (hipEvent_t& e : c->seq_ev) {
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
             (void)hipGetLastError();
@@ -1973,14 +2030,17 @@ int sphe_set_list_capacity(sphe_ctx* c, int entries)
     if (!c) return SPHE_EARG;
     if (entries < 16 || entries > (1 << 20) || entries % 16)
         return fail(c, SPHE_EARG, "list capacity must be a multiple of 16 in [16, 1048576]");
+    // the overflow chunks take the same size; their count is the initial one
     if (!c->layout_valid) {
         c->maxn = entries;
+        c->ocap = entries;
         return SPHE_OK;
     }
-    if (entries == c->D.maxn) return SPHE_OK;
-    const int r = resize_lists(c, entries);
+    if (entries == c->D.maxn && entries == c->D.ocap) return SPHE_OK;
+    const int r = resize_lists(c, entries, entries, pool_chunks_initial(c->D.P));
     if (r) return r;
-    if (c->D.maxn != entries) return fail(c, SPHE_ENOMEM, "no device memory for lists of %d entries", entries);
+    if (c->D.maxn != entries || c->D.ocap != entries)
+        return fail(c, SPHE_ENOMEM, "no device memory for lists of %d entries", entries);
     return SPHE_OK;
 }
 
@@ -1988,6 +2048,15 @@ int sphe_get_list_capacity(const sphe_ctx* c, int* entries)
 {
     if (!c || !entries) return SPHE_EARG;
     *entries = c->layout_valid ? c->D.maxn : c->maxn;
+    return SPHE_OK;
+}
+
+int sphe_get_overflow_capacity(const sphe_ctx* c, int* chunk_entries, int64_t* chunks)
+{
+    if (!c) return SPHE_EARG;
+    const int oc = c->layout_valid ? c->D.ocap : c->ocap;
+    if (chunk_entries) *chunk_entries = oc;
+    if (chunks) *chunks = c->layout_valid ? c->D.pool_cap / oc : 0;
     return SPHE_OK;
 }
 

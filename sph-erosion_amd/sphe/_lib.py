@@ -107,6 +107,7 @@ SIGNATURES = {
     "sphe_restore_state": (I, [P]),
     "sphe_set_list_capacity": (I, [P, I]),
     "sphe_get_list_capacity": (I, [P, C.POINTER(I)]),
+    "sphe_get_overflow_capacity": (I, [P, C.POINTER(I), C.POINTER(C.c_int64)]),
     "sphe_set_subcells": (I, [P, I]),
     "sphe_get_subcells": (I, [P, C.POINTER(I)]),
     "sphe_set_yz_split": (I, [P, I]),

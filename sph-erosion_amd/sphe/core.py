@@ -337,6 +337,12 @@ class Context:
         self._chk(lib().sphe_get_list_capacity(self._p, C.byref(v)), "get_list_capacity")
         return v.value
 
+    def get_overflow_capacity(self):
+        """(entries per overflow chunk, chunks in the pool)."""
+        e, n = C.c_int(), C.c_int64()
+        self._chk(lib().sphe_get_overflow_capacity(self._p, C.byref(e), C.byref(n)), "get_overflow_capacity")
+        return e.value, n.value
+
     def set_subcells(self, slabs):
         self._chk(lib().sphe_set_subcells(self._p, slabs), "set_subcells")
 

@@ -412,10 +412,14 @@ def test_list_overflow_paths_match_the_twin():
     for _ in range(2):
         st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], np.arange(n, dtype=np.int32), real=4, comp=1)
     gp, gv = ctx.state_f64(sid)
-    assert ctx.status() == 0
+    assert ctx.status() == 4  # particles past list and pool walked their stencil
     assert np.array_equal(gp, st["pos"]) and np.array_equal(gv, st["vel"])
-    ctx.step(1)  # the overflow grew the lists for the following steps
-    assert ctx.get_list_capacity() >= 1.5 * counts.max()
+    ctx.step(1)  # the overflow grew the lists (and the chunks) for the following steps
+    cap = ctx.get_list_capacity()
+    chunk, chunks = ctx.get_overflow_capacity()
+    assert cap % 128 == 0 and cap >= counts.max() and cap + chunk >= counts.max() and chunks >= 64
+    ctx.step(1)
+    assert ctx.status() == 0  # every neighbourhood fits now
 
 
 def test_list_capacity_never_changes_results():
