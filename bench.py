@@ -124,6 +124,9 @@ def context_from(scs, states, precision, tuning):
         c.set_particles(s, st["pos"], vel=st["vel"], ids=st["ids"], sed=st["sed"])
         if st["vol"] is not None:
             c.set_terrain_volume_fx(s, st["vol"])
+    c.set_yz_split(tuning[3])
+    c.set_subcells(tuning[4])
+    c.set_scan_tile(tuning[5])
     c.set_tuning(tuning[0], tuning[1])
     c.set_force_block(tuning[2])
     return c, sids
@@ -387,6 +390,7 @@ def main():
     ap.add_argument("--load-state", default=None,
                     help="start from a state written by --save-state (no settle; give --warmup 0 for the same "
                          "window): lets a profiler see only the window's kernels")
+    ap.add_argument("--grid", default=None, help="y/z split, x slabs, log2 scan tile, e.g. 2,4,12 (with --no-tune)")
     ap.add_argument("--block", type=int, default=None, help="density block size (with --no-tune)")
     ap.add_argument("--force-block", type=int, default=None, help="force block size (with --no-tune)")
     a = ap.parse_args()
@@ -413,6 +417,16 @@ def main():
 
     ctx = sphe.Context(sphe.FP32)
     sids = [ctx.add_synthetic(s) for s in scs]
+    if a.no_tune:
+        if a.grid:
+            yz, slabs, tile = (int(x) for x in a.grid.split(","))
+            ctx.set_yz_split(yz)
+            ctx.set_subcells(slabs)
+            ctx.set_scan_tile(tile)
+        if a.block:
+            ctx.set_tuning(a.block, ctx.get_tuning()[1])
+        if a.force_block:
+            ctx.set_force_block(a.force_block)
     if a.load_state:
         load_states(ctx, sids, a.load_state)
         with np.load(a.load_state) as d:
@@ -426,21 +440,20 @@ def main():
             ctx.sync()
         ctx.restore_state()
         log(f"bench: loaded the state after {settle} steps from {a.load_state}")
-    if not a.no_tune:
-        blk, rm = ctx.autotune(5)
-        log(f"bench: autotuned block={blk} force_block={ctx.get_force_block()} xcd_remap={rm}")
-    elif a.block:
-        ctx.set_tuning(a.block, ctx.get_tuning()[1])
-    if a.no_tune and a.force_block:
-        ctx.set_force_block(a.force_block)
-    blk, rm = ctx.get_tuning()
-    fblk = ctx.get_force_block()
-    tuning = (blk, rm, fblk)
     if not a.load_state:
         for done in range(0, settle, 250):
             ctx.step(min(250, settle - done))
             ctx.sync()
             log(f"bench: settled {min(done + 250, settle)} of {settle} steps")
+    if not a.no_tune:
+        # on the settled state: the grid first, then the block sizes on it
+        yz, slabs, tile = ctx.autotune_grid(3)
+        blk, rm = ctx.autotune(5)
+        log(f"bench: autotuned yz_split={yz} x_slabs={slabs} scan_tile=2^{tile} block={blk} "
+            f"force_block={ctx.get_force_block()} xcd_remap={rm}")
+    blk, rm = ctx.get_tuning()
+    fblk = ctx.get_force_block()
+    tuning = (blk, rm, fblk, ctx.get_yz_split(), ctx.get_subcells(), ctx.get_scan_tile())
     ctx.step(a.warmup)
     ctx.sync()
     if a.save_state:
@@ -505,7 +518,7 @@ def main():
                                   f"({settle} settle + {a.warmup} warm-up steps before it, untimed)",
                            precision="fp32 (Kahan-compensated density and force sums)", block=blk,
                            force_block=fblk, density_xcd_run=rm, x_slabs=ctx.get_subcells(),
-                           yz_split=ctx.get_yz_split()),
+                           yz_split=ctx.get_yz_split(), scan_tile_log2=ctx.get_scan_tile()),
                roofline=roofline(dom, kernels[dom], counters))
     out["roofline_force"] = roofline("force", kernels["force"], counters)
     out["value_kernel_by_kernel"] = N * a.steps / prof_elapsed

@@ -339,6 +339,22 @@ int sphe_get_list_capacity(const sphe_ctx* ctx, int* entries);
  * context has a device layout). */
 int sphe_get_overflow_capacity(const sphe_ctx* ctx, int* chunk_entries, int64_t* chunks);
 int sphe_autotune(sphe_ctx* ctx, int steps_per_trial);
+/* Cells per scan tile of the sort's prefix sum: 2^log2_cells, 11..13
+ * (default 12).  Results do not depend on it.  No counterpart in the
+ * reference (which has no grid). */
+int sphe_set_scan_tile(sphe_ctx* ctx, int log2_cells);
+int sphe_get_scan_tile(const sphe_ctx* ctx, int* log2_cells);
+/* Tunes the cell grid on the current state: every y/z split (1, 2) and
+ * x-slab count (1, 2, 4), then every scan tile on the best of them, each
+ * timed over steps_per_trial steps (all kernels, after two untimed steps)
+ * from the same state; keeps the fastest and leaves the state as it was.
+ * The split and the slab count set the canonical summation order, so the
+ * tuned grid changes the last bits of later steps (every combination is a
+ * tested configuration).  The choice is cached per (device, precision,
+ * lanes, block sizes, scene sizes and smoothing lengths).  Run before
+ * sphe_autotune, whose block sizes depend on the grid.  No counterpart in
+ * the reference. */
+int sphe_autotune_grid(sphe_ctx* ctx, int steps_per_trial);
 /* Saves the particle and terrain state of every scene (a host copy) and
  * restores it later; stepping a restored state repeats the same steps bit
  * for bit.  No counterpart in the reference (whose state is the particle

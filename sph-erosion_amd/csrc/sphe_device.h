@@ -15,8 +15,10 @@
 #include <stdint.h>
 
 #define SPHE_WAVE 64
-// cell counts per scan tile (256 threads x 16 counts)
-#define SPHE_SCAN_TILE_LOG2 12
+// cells per scan tile: 2^scan_log2 (256 threads x 8, 16 or 32 counts)
+#define SPHE_SCAN_LOG2_MIN 11
+#define SPHE_SCAN_LOG2_DEFAULT 12
+#define SPHE_SCAN_LOG2_MAX 13
 
 template <typename T>
 struct SceneConsts {
@@ -75,7 +77,7 @@ struct DevState {
     int32_t* key; uint32_t* arank;      // cell key << sub_log2 | x slab, arrival rank in the cell
     uint64_t* tmp_kid; int32_t* tmp_src;  // scatter output: (key, id) packed, previous slot
     uint32_t* cnt; uint32_t* start; uint32_t* partial; int64_t npartial;
-    // one word per scan tile of cnt (SPHE_SCAN_TILE cells): set when a count
+    // one word per scan tile of cnt (2^scan_log2 cells): set when a count
     // in the tile is taken, cleared by the scan that consumes the tile; the
     // scan reads the counts of flagged tiles only (the rest are zero)
     uint32_t* tocc;
@@ -83,6 +85,7 @@ struct DevState {
     // holds when the tile was last written empty, ~0 otherwise; the scan
     // leaves an empty tile unwritten when that value is still right
     uint32_t* tfill;
+    int32_t scan_log2;  // cells per scan tile: 2^scan_log2
     // cells are split into 2^sub_log2 slabs along x (sub_log2 <= 2); subw
     // holds each cell's slab offsets (10 bits per inner boundary; bit 31:
     // more than 1023 particles, no offsets), rewritten every step for the

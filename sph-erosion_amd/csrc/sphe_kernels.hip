@@ -432,7 +432,7 @@ __device__ __forceinline__ void count_position(const DevState& D, const SceneDev
     const int sub = sub_cell<T>(gx, kx, S.dims[0], D.sub_log2);
     D.key[slot] = (int32_t)((key << D.sub_log2) | sub);
     D.arank[slot] = atomicAdd(&D.cnt[key], 1u);
-    D.tocc[key >> SPHE_SCAN_TILE_LOG2] = 1u;
+    D.tocc[key >> D.scan_log2] = 1u;
 }
 
 __device__ __forceinline__ void count_pad(const DevState& D, const SceneDev& S, int64_t slot)
@@ -440,7 +440,7 @@ __device__ __forceinline__ void count_pad(const DevState& D, const SceneDev& S, 
     const int64_t key = S.cell_off + S.ncell;
     D.key[slot] = (int32_t)(key << D.sub_log2);
     D.arank[slot] = atomicAdd(&D.cnt[key], 1u);
-    D.tocc[key >> SPHE_SCAN_TILE_LOG2] = 1u;
+    D.tocc[key >> D.scan_log2] = 1u;
 }
 
 // ------------------------------------------------------------------ sort
@@ -460,9 +460,8 @@ __global__ __launch_bounds__(256) void k_hash_count(DevState D)
     }
 }
 
-#define SCAN_ITEMS 16
-#define SCAN_TILE (256 * SCAN_ITEMS)
-static_assert(SCAN_TILE == (1 << SPHE_SCAN_TILE_LOG2), "scan tile");
+// A scan tile is 256 threads x ITEMS consecutive counts (ITEMS = 8, 16 or
+// 32: tiles of 2^11, 2^12 or 2^13 cells, DevState::scan_log2).
 
 // Exclusive scan of v over a 256-thread block, with its total, and the
 // block total of a second value w: one LDS exchange (lds: 8 words).
@@ -496,39 +495,42 @@ __device__ __forceinline__ uint32_t block_scan_sum(uint32_t v, uint32_t w, uint3
     return base + x - v;
 }
 
-// SCAN_ITEMS consecutive words per thread, as 16-byte accesses where the
-// whole run is inside [0, n) (tiles start at multiples of 64 bytes).
+// ITEMS consecutive words per thread, as 16-byte accesses where the whole
+// run is inside [0, n) (tiles start at multiples of 64 bytes).
+template <int ITEMS>
 __device__ __forceinline__ void load_run(const uint32_t* __restrict__ p, int64_t base, int64_t n,
                                          uint32_t* v)
 {
-    if (base + SCAN_ITEMS <= n) {
+    if (base + ITEMS <= n) {
         const uint4* q = (const uint4*)(p + base);
 #pragma unroll
-        for (int k = 0; k < SCAN_ITEMS / 4; ++k) {
+        for (int k = 0; k < ITEMS / 4; ++k) {
             const uint4 a = q[k];
             v[4 * k] = a.x; v[4 * k + 1] = a.y; v[4 * k + 2] = a.z; v[4 * k + 3] = a.w;
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < SCAN_ITEMS; ++i) v[i] = base + i < n ? p[base + i] : 0u;
+        for (int i = 0; i < ITEMS; ++i) v[i] = base + i < n ? p[base + i] : 0u;
     }
 }
 
+template <int ITEMS>
 __device__ __forceinline__ void store_run(uint32_t* __restrict__ p, int64_t base, int64_t n, const uint32_t* v)
 {
-    if (base + SCAN_ITEMS <= n) {
+    if (base + ITEMS <= n) {
         uint4* q = (uint4*)(p + base);
 #pragma unroll
-        for (int k = 0; k < SCAN_ITEMS / 4; ++k) q[k] = make_uint4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+        for (int k = 0; k < ITEMS / 4; ++k) q[k] = make_uint4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
     } else {
 #pragma unroll
-        for (int i = 0; i < SCAN_ITEMS; ++i)
+        for (int i = 0; i < ITEMS; ++i)
             if (base + i < n) p[base + i] = v[i];
     }
 }
 
 // Total of count tile `tile` into partial[tile].  occ (optional): the tile
 // flags; an unflagged tile's counts are all zero.
+template <int ITEMS>
 __device__ __forceinline__ void scan_reduce_tile(const uint32_t* __restrict__ in, int64_t n,
                                                  uint32_t* __restrict__ partial, const uint32_t* __restrict__ occ,
                                                  int tile, uint32_t* lds)
@@ -537,23 +539,24 @@ __device__ __forceinline__ void scan_reduce_tile(const uint32_t* __restrict__ in
         if (threadIdx.x == 0) partial[tile] = 0u;
         return;
     }
-    const int64_t base = (int64_t)tile * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-    uint32_t v[SCAN_ITEMS];
-    load_run(in, base, n, v);
+    const int64_t base = (int64_t)tile * (256 * ITEMS) + (int64_t)threadIdx.x * ITEMS;
+    uint32_t v[ITEMS];
+    load_run<ITEMS>(in, base, n, v);
     uint32_t s = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i) s += v[i];
+    for (int i = 0; i < ITEMS; ++i) s += v[i];
     uint32_t tot, none;
     (void)block_scan_sum(0u, s, &none, &tot, lds);
     if (threadIdx.x == 0) partial[tile] = tot;
 }
 
+template <int ITEMS>
 __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
                                                      uint32_t* __restrict__ partial,
                                                      const uint32_t* __restrict__ occ)
 {
     __shared__ uint32_t lds[8];
-    scan_reduce_tile(in, n, partial, occ, (int)blockIdx.x, lds);
+    scan_reduce_tile<ITEMS>(in, n, partial, occ, (int)blockIdx.x, lds);
 }
 
 // start[i] = exclusive prefix of counts.  With zero_counts the counts are
@@ -564,6 +567,7 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
 // fill (optional, with occ): per tile, the value all of an empty tile's
 // starts hold since the tile was last written empty (~0: not known); an
 // empty tile whose prefix equals it is not written again.
+template <int ITEMS>
 __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, uint32_t* __restrict__ start,
                                                    int64_t n, const uint32_t* __restrict__ partial,
                                                    int zero_counts, unsigned long long* __restrict__ zero,
@@ -571,15 +575,15 @@ __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, u
 {
     if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0ull;
     __shared__ uint32_t lds[8];
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    const int64_t base = (int64_t)blockIdx.x * (256 * ITEMS) + (int64_t)threadIdx.x * ITEMS;
     const bool empty = occ && occ[blockIdx.x] == 0u;  // block-uniform
     const uint32_t filled = fill ? fill[blockIdx.x] : ~0u;
-    uint32_t v[SCAN_ITEMS];
+    uint32_t v[ITEMS];
     if (empty) {
 #pragma unroll
-        for (int i = 0; i < SCAN_ITEMS; ++i) v[i] = 0u;
+        for (int i = 0; i < ITEMS; ++i) v[i] = 0u;
     } else {
-        load_run(cnt, base, n, v);
+        load_run<ITEMS>(cnt, base, n, v);
     }
     // the totals of the tiles before this one, eight loads in flight per thread
     uint32_t prior = 0;
@@ -595,13 +599,13 @@ __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, u
     }
     uint32_t s = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i) s += v[i];
+    for (int i = 0; i < ITEMS; ++i) s += v[i];
     uint32_t tot, ptot;
     uint32_t run = block_scan_sum(s, prior, &tot, &ptot, lds);
     run += ptot;
-    uint32_t z[SCAN_ITEMS];
+    uint32_t z[ITEMS];
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i) {
+    for (int i = 0; i < ITEMS; ++i) {
         const uint32_t c = v[i];
         v[i] = run;
         run += c;
@@ -609,8 +613,8 @@ __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, u
     }
     // an empty tile's starts all equal ptot (its prefix): unwritten when they
     // already hold it
-    if (!(empty && filled == ptot)) store_run(start, base, n, v);
-    if (zero_counts && !empty) store_run(cnt, base, n, z);
+    if (!(empty && filled == ptot)) store_run<ITEMS>(start, base, n, v);
+    if (zero_counts && !empty) store_run<ITEMS>(cnt, base, n, z);
     // every thread read the flag and the fill word before the barrier of the
     // scan above
     if (threadIdx.x == 0) {
@@ -1566,6 +1570,7 @@ __global__ __launch_bounds__(256) void k_heights(DevState D)
 // (both need only the force pass done): blocks [0, nscan) total the count
 // tiles, the others compute heights from the volumes the last force pass
 // eroded, before this step's force pass samples them.
+template <int ITEMS>
 __global__ __launch_bounds__(256) void k_scan_reduce_heights(DevState D, int nscan)
 {
     __shared__ uint32_t lds[8];
@@ -1574,7 +1579,7 @@ __global__ __launch_bounds__(256) void k_scan_reduce_heights(DevState D, int nsc
         if (t0 < D.TC) heights_quad(D, t0);
         return;
     }
-    scan_reduce_tile(D.cnt, D.C + 1, D.partial, D.tocc, (int)blockIdx.x, lds);
+    scan_reduce_tile<ITEMS>(D.cnt, D.C + 1, D.partial, D.tocc, (int)blockIdx.x, lds);
 }
 
 // Two-phase erosion, phase 2a: count exchanges per terrain cell.
@@ -2005,18 +2010,28 @@ void sphe_launch_hash(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent
     else klaunch(k_hash_count<float>, grid_of(D.P, 256), 256, st, e0, e1, D);
 }
 
-void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, bool zero_counts,
-                      unsigned long long* zero, uint32_t* occ, uint32_t* fill, hipStream_t st, hipEvent_t e0,
-                      hipEvent_t e1)
+template <int ITEMS>
+static void launch_scan_t(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, bool zero_counts,
+                          unsigned long long* zero, uint32_t* occ, uint32_t* fill, hipStream_t st, hipEvent_t e0,
+                          hipEvent_t e1)
 {
-    const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    klaunch(k_scan_reduce, (unsigned)nb, 256, st, e0, nullptr, (const uint32_t*)cnt, n, partial,
+    const int64_t nb = (n + 256 * ITEMS - 1) / (256 * ITEMS);
+    klaunch(k_scan_reduce<ITEMS>, (unsigned)nb, 256, st, e0, nullptr, (const uint32_t*)cnt, n, partial,
             (const uint32_t*)occ);
-    klaunch(k_scan_down, (unsigned)nb, 256, st, nullptr, e1, cnt, start, n, (const uint32_t*)partial,
+    klaunch(k_scan_down<ITEMS>, (unsigned)nb, 256, st, nullptr, e1, cnt, start, n, (const uint32_t*)partial,
             zero_counts ? 1 : 0, zero, occ, occ ? fill : nullptr);
 }
 
-int64_t sphe_scan_partials(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, bool zero_counts,
+                      unsigned long long* zero, uint32_t* occ, uint32_t* fill, int scan_log2, hipStream_t st,
+                      hipEvent_t e0, hipEvent_t e1)
+{
+    if (scan_log2 == 11) launch_scan_t<8>(cnt, start, n, partial, zero_counts, zero, occ, fill, st, e0, e1);
+    else if (scan_log2 == 13) launch_scan_t<32>(cnt, start, n, partial, zero_counts, zero, occ, fill, st, e0, e1);
+    else launch_scan_t<16>(cnt, start, n, partial, zero_counts, zero, occ, fill, st, e0, e1);
+}
+
+int64_t sphe_scan_partials(int64_t n) { return (n + (1 << SPHE_SCAN_LOG2_MIN) - 1) >> SPHE_SCAN_LOG2_MIN; }
 
 void sphe_launch_scatter(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
@@ -2082,14 +2097,22 @@ void sphe_launch_force(const DevState& D, int block, int remap, int lanes, hipSt
     }
 }
 
-void sphe_launch_scan_heights(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+template <int ITEMS>
+static void launch_scan_heights_t(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     const int64_t n = D.C + 1;
-    const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    const int64_t nb = (n + 256 * ITEMS - 1) / (256 * ITEMS);
     const int64_t nh = (int64_t)grid_of(D.TC / 4, 256);  // TC: a multiple of 64
-    klaunch(k_scan_reduce_heights, (unsigned)(nb + nh), 256, st, e0, nullptr, D, (int)nb);
-    klaunch(k_scan_down, (unsigned)nb, 256, st, nullptr, e1, D.cnt, D.start, n, (const uint32_t*)D.partial, 0,
-            D.ptop, D.tocc, D.tfill);
+    klaunch(k_scan_reduce_heights<ITEMS>, (unsigned)(nb + nh), 256, st, e0, nullptr, D, (int)nb);
+    klaunch(k_scan_down<ITEMS>, (unsigned)nb, 256, st, nullptr, e1, D.cnt, D.start, n, (const uint32_t*)D.partial,
+            0, D.ptop, D.tocc, D.tfill);
+}
+
+void sphe_launch_scan_heights(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+{
+    if (D.scan_log2 == 11) launch_scan_heights_t<8>(D, st, e0, e1);
+    else if (D.scan_log2 == 13) launch_scan_heights_t<32>(D, st, e0, e1);
+    else launch_scan_heights_t<16>(D, st, e0, e1);
 }
 
 void sphe_launch_heights(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
@@ -2102,7 +2125,8 @@ void sphe_launch_two_phase(const DevState& D, hipStream_t st, hipEvent_t e0, hip
 {
     if (D.TC == 0 || D.P == 0) return;
     klaunch(k_tcount, grid_of(D.P, 256), 256, st, e0, nullptr, D);
-    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, true, nullptr, nullptr, nullptr, st);
+    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, true, nullptr, nullptr, nullptr,
+                     SPHE_SCAN_LOG2_DEFAULT, st);
     k_tscatter<<<grid_of(D.P, 256), 256, 0, st>>>(D);
     klaunch(k_tapply, grid_of(D.TC, 256), 256, st, nullptr, e1, D);
 }

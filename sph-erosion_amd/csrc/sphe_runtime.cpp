@@ -113,7 +113,9 @@ struct sphe_ctx {
     int maxn = 256;  // neighbour-list capacity per particle (grown when a step overflows it)
     int ocap = 256;  // entries per overflow-pool chunk
     size_t mem_total = 0;  // the device's memory (bounds the neighbour lists)
+    int64_t pool_chunks = 0;  // overflow chunks the last growth asked for (0: none yet)
     int lanes = 4;   // lanes per particle in the density and force passes
+    int scan_log2 = SPHE_SCAN_LOG2_DEFAULT;  // cells per scan tile: 2^scan_log2
     int sub_log2 = 2;  // x slabs per cell: 2^sub_log2
     int yz = 2;        // cells are 1/yz of the x edge along y and z
 
@@ -198,9 +200,13 @@ int64_t pool_entries(int64_t chunks, int ocap)
     return std::min<int64_t>(chunks * ocap, (int64_t)INT32_MAX);
 }
 
-// Overflow chunks of a new layout: one for an eighth of the particles (at
-// least 64).
-int64_t pool_chunks_initial(int64_t P) { return std::max<int64_t>(P / 8, 64); }
+// Overflow chunks of a new layout before any step has claimed one: one for
+// an eighth of the particle slots, but at most 16 entries per slot in
+// chunks of ocap entries (at least 64 chunks).
+int64_t pool_chunks_initial(int64_t P, int ocap)
+{
+    return std::max<int64_t>(std::min<int64_t>(P / 8, P * 16 / ocap), 64);
+}
 
 // ---------------------------------------------------------------- derive
 
@@ -509,6 +515,7 @@ int build_layout(sphe_ctx* c)
     DALLOC(D.subw, (size_t)(C + 1) * 4);
     D.yz = c->yz;
     D.sub_log2 = c->sub_log2;
+    D.scan_log2 = c->scan_log2;
     while (D.sub_log2 > 0 && (C << D.sub_log2) > INT32_MAX) --D.sub_log2;  // keys are int32
     D.npartial = sphe_scan_partials(C + 1);
     DALLOC(D.partial, (size_t)D.npartial * 4);
@@ -527,7 +534,8 @@ int build_layout(sphe_ctx* c)
     DALLOC(D.nbl, Pn * (size_t)D.maxn * 4);
     DALLOC(D.nbc, Pn * 4);
     D.ocap = c->ocap;
-    D.pool_cap = pool_entries(pool_chunks_initial(P), D.ocap);
+    // a rebuilt layout keeps the pool the steps before it sized
+    D.pool_cap = pool_entries(c->pool_chunks > 0 ? c->pool_chunks : pool_chunks_initial(P, D.ocap), D.ocap);
     DALLOC(D.pool, (size_t)D.pool_cap * 4);
     DALLOC(D.pbase, Pn * 4);
     DALLOC(D.ptop, 64);
@@ -820,8 +828,8 @@ void launch_sort(sphe_ctx* c)
         // with erosion the terrain heights of the last force pass are computed
         // in the scan's first launch (this step's force pass samples them)
         if (any_erosion(c, -1) && D.TC > 0) sphe_launch_scan_heights(D, c->stream, t.a, t.b);
-        else sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, false, D.ptop, D.tocc, D.tfill, c->stream, t.a,
-                              t.b);
+        else sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, false, D.ptop, D.tocc, D.tfill, D.scan_log2,
+                              c->stream, t.a, t.b);
     }
     {
         Timed t(c, "scatter");
@@ -854,12 +862,13 @@ void launch_step(sphe_ctx* c)
 std::string plan_signature(const sphe_ctx* c)
 {
     // everything the captured kernels take by value (DevState) or as launch shape
-    char b[320];
-    snprintf(b, sizeof(b), "%d/%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%p/%d/%d/%d/%d/%lld", c->precision, c->block,
+    char b[352];
+    snprintf(b, sizeof(b), "%d/%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%p/%d/%d/%d/%d/%lld/%d", c->precision, c->block,
              force_block(c), c->remap,
              c->lanes, (long long)c->D.P, (long long)c->D.C, (long long)c->D.TC, (int)any_erosion(c, -1),
              (int)any_erosion(c, SPHE_EROSION_TWO_PHASE), (const void*)c->D.u_pos, (const void*)c->D.nbl,
-             (const void*)c->D.pool, c->D.maxn, c->D.sub_log2, c->D.yz, c->D.ocap, (long long)c->D.pool_cap);
+             (const void*)c->D.pool, c->D.maxn, c->D.sub_log2, c->D.yz, c->D.ocap, (long long)c->D.pool_cap,
+             c->D.scan_log2);
     return b;
 }
 
@@ -952,6 +961,7 @@ int resize_lists(sphe_ctx* c, int maxn, int ocap, int64_t chunks)
     if (ok) c->D.ocap = ocap;
     c->maxn = c->D.maxn;
     c->ocap = c->D.ocap;
+    c->pool_chunks = c->D.pool_cap / c->D.ocap;
     ++c->epoch;  // the new lists hold nothing yet
     return SPHE_OK;
 }
@@ -2034,10 +2044,11 @@ int sphe_set_list_capacity(sphe_ctx* c, int entries)
     if (!c->layout_valid) {
         c->maxn = entries;
         c->ocap = entries;
+        c->pool_chunks = 0;
         return SPHE_OK;
     }
     if (entries == c->D.maxn && entries == c->D.ocap) return SPHE_OK;
-    const int r = resize_lists(c, entries, entries, pool_chunks_initial(c->D.P));
+    const int r = resize_lists(c, entries, entries, pool_chunks_initial(c->D.P, entries));
     if (r) return r;
     if (c->D.maxn != entries || c->D.ocap != entries)
         return fail(c, SPHE_ENOMEM, "no device memory for lists of %d entries", entries);
@@ -2105,6 +2116,29 @@ int sphe_get_yz_split(const sphe_ctx* c, int* split)
 {
     if (!c || !split) return SPHE_EARG;
     *split = c->yz;
+    return SPHE_OK;
+}
+
+int sphe_set_scan_tile(sphe_ctx* c, int log2_cells)
+{
+    if (!c) return SPHE_EARG;
+    if (log2_cells < SPHE_SCAN_LOG2_MIN || log2_cells > SPHE_SCAN_LOG2_MAX)
+        return fail(c, SPHE_EARG, "scan tile must be 2^%d .. 2^%d cells", SPHE_SCAN_LOG2_MIN, SPHE_SCAN_LOG2_MAX);
+    c->scan_log2 = log2_cells;
+    if (!c->layout_valid || c->D.scan_log2 == log2_cells) return SPHE_OK;
+    // the tile flags and fill words describe tiles of the old size: every
+    // tile is read and written by the next scan, which sets them up again
+    HIPCHK(c, hipMemsetAsync(c->D.tocc, 0x01, (size_t)c->D.npartial * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->D.tfill, 0xFF, (size_t)c->D.npartial * 4, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->D.scan_log2 = log2_cells;
+    return SPHE_OK;
+}
+
+int sphe_get_scan_tile(const sphe_ctx* c, int* log2_cells)
+{
+    if (!c || !log2_cells) return SPHE_EARG;
+    *log2_cells = c->layout_valid ? c->D.scan_log2 : c->scan_log2;
     return SPHE_OK;
 }
 
@@ -2230,6 +2264,128 @@ int sphe_autotune(sphe_ctx* c, int steps_per_trial)
         std::lock_guard<std::mutex> g(mu);
         cache[key] = {c->block, c->fblock, c->remap};
     }
+    return SPHE_OK;
+}
+
+// One trial of the grid autotuner: the step from the state in host staging
+// (one untimed step that also grows the lists, then `steps` profiled
+// steps); *ms = the kernels' time per step.
+static int grid_trial(sphe_ctx* c, int steps, double* ms)
+{
+    c->host_valid = true;     // staging holds the state every trial starts from
+    c->device_valid = false;
+    c->profiling = false;
+    int r = do_steps(c, 2);
+    if (r == SPHE_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = fail(c, SPHE_EHIP, "autotune sync");
+    if (r) return r;
+    c->host_valid = true;
+    c->device_valid = false;
+    c->prof.clear();
+    c->profiling = true;
+    r = do_steps(c, steps);
+    if (r == SPHE_OK) r = collect_profile(c);
+    c->profiling = false;
+    if (r) return r;
+    double t = 0.0;
+    for (const auto& kv : c->prof) t += kv.second.first;
+    *ms = t / steps;
+    return SPHE_OK;
+}
+
+int sphe_autotune_grid(sphe_ctx* c, int steps_per_trial)
+{
+    if (!c) return SPHE_EARG;
+    if (steps_per_trial < 1) steps_per_trial = 1;
+    static std::mutex mu;
+    struct Tuned {
+        int yz, slabs, scan_log2;
+    };
+    static std::map<std::string, Tuned> cache;
+    int r = prepare(c);
+    if (r) return r;
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    // what the choice depends on: the device, the kernels' variants, the
+    // scenes' sizes and smoothing lengths (not the grid being tuned)
+    std::string key = std::string(prop.gcnArchName) + "|" + std::to_string(c->precision) + "/" +
+                      std::to_string(c->lanes) + "/" + std::to_string(c->block) + "/" +
+                      std::to_string(force_block(c)) + "/" + std::to_string(c->D.P) + "/" +
+                      std::to_string((int)any_erosion(c, -1));
+    for (const Scene& s : c->scenes)
+        key += "|" + std::to_string(s.ids.size()) + ":" + std::to_string(s.prm.h) + ":" +
+               std::to_string(s.prm.box_half) + ":" + std::to_string(s.tnx) + "x" + std::to_string(s.tnz);
+    Tuned best{c->yz, 1 << c->D.sub_log2, c->D.scan_log2};
+    // every trial, and the context afterwards, starts from the state as it is now
+    r = sync_to_host(c);
+    if (r) return r;
+    bool cached = false;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            best = it->second;
+            cached = true;
+        }
+    }
+    if (!cached) {
+        r = collect_profile(c);
+        if (r) return r;
+        const bool prof = c->profiling;
+        const auto saved_prof = c->prof;
+        double best_ms = 1e30;
+        auto restore = [&](int rc) {
+            c->prof = saved_prof;
+            c->profiling = prof;
+            return rc;
+        };
+        // the cell grid: y/z split and x slabs (the canonical order changes
+        // with them; every combination is a tested configuration)
+        for (int yz : {2, 1}) {
+            for (int slabs : {4, 2, 1}) {
+                c->host_valid = true;  // a new split rebuilds the layout from the staged state
+                r = sphe_set_yz_split(c, yz);
+                if (r == SPHE_OK) r = prepare(c);
+                if (r == SPHE_OK) r = sphe_set_subcells(c, slabs);
+                double ms = 0.0;
+                if (r == SPHE_OK) r = grid_trial(c, steps_per_trial, &ms);
+                if (r) return restore(r);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best.yz = yz;
+                    best.slabs = slabs;
+                }
+            }
+        }
+        c->host_valid = true;
+        r = sphe_set_yz_split(c, best.yz);
+        if (r == SPHE_OK) r = prepare(c);
+        if (r == SPHE_OK) r = sphe_set_subcells(c, best.slabs);
+        if (r) return restore(r);
+        // the scan tile on the chosen grid
+        best_ms = 1e30;
+        for (int t = SPHE_SCAN_LOG2_MIN; t <= SPHE_SCAN_LOG2_MAX; ++t) {
+            r = sphe_set_scan_tile(c, t);
+            double ms = 0.0;
+            if (r == SPHE_OK) r = grid_trial(c, steps_per_trial, &ms);
+            if (r) return restore(r);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.scan_log2 = t;
+            }
+        }
+        restore(SPHE_OK);
+        std::lock_guard<std::mutex> g(mu);
+        cache[key] = best;
+    }
+    c->host_valid = true;
+    r = sphe_set_yz_split(c, best.yz);
+    if (r == SPHE_OK) r = prepare(c);
+    if (r == SPHE_OK) r = sphe_set_subcells(c, best.slabs);
+    if (r == SPHE_OK) r = sphe_set_scan_tile(c, best.scan_log2);
+    if (r) return r;
+    c->host_valid = true;     // host staging still holds the pre-tuning state
+    c->device_valid = false;  // restore it on the next use
+    c->stepped = false;
     return SPHE_OK;
 }
 

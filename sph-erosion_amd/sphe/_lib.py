@@ -101,6 +101,9 @@ SIGNATURES = {
     "sphe_set_force_block": (I, [P, I]),
     "sphe_get_force_block": (I, [P, C.POINTER(I)]),
     "sphe_autotune": (I, [P, I]),
+    "sphe_autotune_grid": (I, [P, I]),
+    "sphe_set_scan_tile": (I, [P, I]),
+    "sphe_get_scan_tile": (I, [P, C.POINTER(I)]),
     "sphe_set_lanes": (I, [P, I]),
     "sphe_get_lanes": (I, [P, C.POINTER(I)]),
     "sphe_save_state": (I, [P]),

@@ -365,6 +365,19 @@ class Context:
         self._chk(lib().sphe_autotune(self._p, steps_per_trial), "autotune")
         return self.get_tuning()
 
+    def autotune_grid(self, steps_per_trial=3):
+        """Tunes the cell grid; returns (y/z split, x slabs, log2 cells per scan tile)."""
+        self._chk(lib().sphe_autotune_grid(self._p, steps_per_trial), "autotune_grid")
+        return self.get_yz_split(), self.get_subcells(), self.get_scan_tile()
+
+    def set_scan_tile(self, log2_cells):
+        self._chk(lib().sphe_set_scan_tile(self._p, log2_cells), "set_scan_tile")
+
+    def get_scan_tile(self):
+        v = C.c_int()
+        self._chk(lib().sphe_get_scan_tile(self._p, C.byref(v)), "get_scan_tile")
+        return v.value
+
     # ---- verification hooks
     def debug_cell_keys(self, scene):
         out = np.zeros(self.num_particles(scene), dtype=np.int32)

@@ -327,6 +327,55 @@ def test_yz_split_bit_exact_twin(yz):
     assert np.array_equal(ctx.sediment_fx(sid), st["sed"])
 
 
+def test_autotuned_grid_steps_bit_exact_with_the_twin():
+    """The grid autotuner leaves the state as it was, and its choice of
+    split and slabs steps bit-exact with the twin configured the same way;
+    a second call takes the cached choice."""
+    sc = small_dam()
+    ctx, sid = gpu_context(sc)
+    pos, vel, sed, vol, height, fx = oracle_inputs(ctx, sid, sc)
+    yz, slabs, tile = ctx.autotune_grid(2)
+    assert yz in (1, 2) and slabs in (1, 2, 4) and 11 <= tile <= 13
+    gp0, gv0 = ctx.state_f64(sid)
+    assert np.array_equal(gp0, pos) and np.array_equal(gv0, vel)
+    assert ctx.autotune_grid(2) == (yz, slabs, tile)
+    osc = orc_scene_from(sc.params, sc.terrain, fx)
+    osc.yz = yz
+    osc.slabs = slabs
+    ctx.step(2)
+    gp, gv = ctx.state_f64(sid)
+    st = dict(pos=pos, vel=vel, sed=sed, vol=vol, height=height)
+    for _ in range(2):
+        st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], sc.ids, st["vol"], st["height"], real=4, comp=1)
+    assert np.array_equal(gp, st["pos"]) and np.array_equal(gv, st["vel"])
+    assert np.array_equal(ctx.sediment_fx(sid), st["sed"])
+
+
+def test_scan_tile_never_changes_results():
+    """Scan tiles of 2^11 and 2^13 cells, set before the first step or between
+    steps (the tile flags and fill words are reset then), step a batch to
+    the same bits as the default tile."""
+    scs = [scenes.dam_break_scene(60 + i, n, frac, T, steps=12)
+           for i, (n, frac, T) in enumerate([(3000, 0.3, 64), (1100, 0.45, 128), (5000, 0.2, 192)])]
+    runs = []
+    for tile, when in ((12, 0), (11, 0), (13, 0), (11, 5), (13, 5)):
+        ctx = sphe.Context()
+        sids = [ctx.add_synthetic(s) for s in scs]
+        if when == 0:
+            ctx.set_scan_tile(tile)
+        ctx.step(5)
+        if when:
+            ctx.set_scan_tile(tile)
+        assert ctx.get_scan_tile() == tile
+        ctx.step(7)
+        assert ctx.status() & 3 == 0
+        runs.append([ctx.state_f64(s) + (ctx.terrain_volume_fx(s),) for s in sids])
+        ctx.close()
+    for r in runs[1:]:
+        for (p, v, t), (p0, v0, t0) in zip(r, runs[0]):
+            assert np.array_equal(p, p0) and np.array_equal(v, v0) and np.array_equal(t, t0)
+
+
 def test_yz_split_change_keeps_the_state():
     """Changing the split after steps rebuilds the layout from the pulled
     state: the following steps equal the twin's with the new split from that
