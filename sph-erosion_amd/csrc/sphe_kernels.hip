@@ -652,28 +652,61 @@ __global__ __launch_bounds__(256) void k_scatter(DevState D)
 // first also writes the cell's slab offsets: for s = 1 .. 2^sub_log2 - 1 the
 // number of the cell's particles in slabs below s, 10 bits each from bit
 // 10 (s - 1); a cell of more than 1023 particles gets bit 31 instead.
+// A particle's rank is counted over its cell's (key, id) words: the block
+// first stages the words of the cells its 256 slots lie in (a window of at
+// most SPHE_REORDER_WINDOW slots) in LDS, with one coalesced pass, so each
+// word is read from memory about once; a block whose cells are larger ranks
+// from memory.
+#define SPHE_REORDER_WINDOW 2048
 template <typename T>
 __global__ __launch_bounds__(256) void k_reorder(DevState D)
 {
+    __shared__ uint64_t wkid[SPHE_REORDER_WINDOW];
+    __shared__ uint32_t wlo, whi;
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= D.P) return;
+    const bool live = d < D.P;
     const int sl = D.sub_log2;
-    const uint64_t kid = D.tmp_kid[d];
-    const int32_t src = D.tmp_src[d];
+    uint64_t kid = 0;
+    int32_t src = 0, c = 0;
+    uint32_t s0 = 0, s1 = 0;
+    bool ok = false;
+    if (live) {
+        kid = D.tmp_kid[d];
+        src = D.tmp_src[d];
+        const int32_t fk = (int32_t)(kid >> 32);
+        c = fk >> sl;
+        ok = fk >= 0 && (int64_t)c < D.C;
+        if (ok) {
+            s0 = D.start[c];
+            s1 = (int64_t)D.start[c + 1] <= D.P ? D.start[c + 1] : (uint32_t)D.P;
+        }
+    }
+    if (threadIdx.x == 0) {
+        wlo = ~0u;
+        whi = 0u;
+    }
+    __syncthreads();
+    if (ok && s0 < s1) {
+        atomicMin(&wlo, s0);
+        atomicMax(&whi, s1);
+    }
+    __syncthreads();
+    const uint32_t w0 = wlo, w1 = whi;
+    const bool staged = w0 < w1 && w1 - w0 <= (uint32_t)SPHE_REORDER_WINDOW;  // block-uniform
+    if (staged)
+        for (uint32_t j = w0 + threadIdx.x; j < w1; j += blockDim.x) wkid[j - w0] = D.tmp_kid[j];
+    __syncthreads();
+    if (!live) return;
+    if (!ok) {
+        atomicOr(D.status, 2u);
+        return;
+    }
     // the particle's state, gathered while its rank is found (from a slot
     // clamped into range: a bad src is reported below and never stored)
     const uint32_t sc = (uint32_t)min(max(src, 0), (int32_t)(D.P - 1));
     const V4<T> pv = at((const V4<T>*)D.u_pos, sc);
     const V4<T> vv = at((const V4<T>*)D.u_vel, sc);
     const int64_t sed = D.u_sed[sc];
-    const int32_t fk = (int32_t)(kid >> 32);
-    const int32_t c = fk >> sl;
-    if (fk < 0 || (int64_t)c >= D.C) {
-        atomicOr(D.status, 2u);
-        return;
-    }
-    const uint32_t s0 = D.start[c];
-    const uint32_t s1 = (int64_t)D.start[c + 1] <= D.P ? D.start[c + 1] : (uint32_t)D.P;
     const int32_t my = (int32_t)((uint32_t)kid ^ 0x80000000u);
     // one count field per slab boundary: a particle in slab k adds 1 to the
     // fields of the boundaries s > k
@@ -685,12 +718,23 @@ __global__ __launch_bounds__(256) void k_reorder(DevState D)
         below += fields & (~0u << (10u * ((uint32_t)(o >> 32) & smask)));
     };
     uint32_t j = s0;
-    for (; j + 1 < s1; j += 2) {  // two loads in flight
-        const uint64_t o0 = D.tmp_kid[j], o1 = D.tmp_kid[j + 1];
-        take(j, o0);
-        take(j + 1, o1);
+    if (staged) {
+        for (; j + 3 < s1; j += 4) {
+            const uint64_t o0 = wkid[j - w0], o1 = wkid[j + 1 - w0], o2 = wkid[j + 2 - w0], o3 = wkid[j + 3 - w0];
+            take(j, o0);
+            take(j + 1, o1);
+            take(j + 2, o2);
+            take(j + 3, o3);
+        }
+        for (; j < s1; ++j) take(j, wkid[j - w0]);
+    } else {
+        for (; j + 1 < s1; j += 2) {  // two loads in flight
+            const uint64_t o0 = D.tmp_kid[j], o1 = D.tmp_kid[j + 1];
+            take(j, o0);
+            take(j + 1, o1);
+        }
+        if (j < s1) take(j, D.tmp_kid[j]);
     }
-    if (j < s1) take(j, D.tmp_kid[j]);
     const uint32_t f = s0 + rank;
     if ((int64_t)f >= D.P || src < 0 || (int64_t)src >= D.P) {
         atomicOr(D.status, 2u);
