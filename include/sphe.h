@@ -339,6 +339,15 @@ int sphe_get_list_capacity(const sphe_ctx* ctx, int* entries);
  * context has a device layout). */
 int sphe_get_overflow_capacity(const sphe_ctx* ctx, int* chunk_entries, int64_t* chunks);
 int sphe_autotune(sphe_ctx* ctx, int steps_per_trial);
+/* Persistent density and force passes (default off): a grid of resident
+ * blocks takes the passes' blocks of every scene from eight work queues,
+ * one per XCD, each keeping the ordinary launch's XCD runs of cells, with
+ * blocks of an empty queue taking work from the others.  Results are the
+ * same bits either way; on MI355X the ordinary launches are faster (the
+ * hardware refills a CU with no atomic per work item).  No counterpart in
+ * the reference. */
+int sphe_set_persistent(sphe_ctx* ctx, int on);
+int sphe_get_persistent(const sphe_ctx* ctx, int* on);
 /* Cells per scan tile of the sort's prefix sum: 2^log2_cells, 11..13
  * (default 12).  Results do not depend on it.  No counterpart in the
  * reference (which has no grid). */

@@ -122,4 +122,8 @@ struct DevState {
     // ovf[0]: largest neighbour count that overflowed a list; ovf[1]:
     // particles that found the pool full (both since the host last grew them)
     uint32_t* ovf;
+    // persistent density and force passes (when set): a resident grid takes
+    // the passes' blocks from eight queues, work[0..7]
+    int32_t persistent;
+    uint32_t* work;
 };

@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <tuple>
+#include <algorithm>
 #include <mutex>
 
 #include "sphe_device.h"
@@ -827,9 +829,8 @@ __host__ __device__ constexpr int density_lds_bytes(int W, int ngroups, int ring
 // group then merges the partial sums and lane 0 writes rho and
 // (p / rho^2, m / rho).
 template <typename T, int SM, int G>
-__global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap)
+__device__ __forceinline__ void density_body(const DevState& D, int b)
 {
-    const int b = xcd_block(blockIdx.x, nblk, remap);
     const int64_t gt = (int64_t)b * blockDim.x + threadIdx.x;
     const int64_t slot = gt / G;
     const int l = (int)(gt & (G - 1));
@@ -1071,6 +1072,12 @@ __global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap
     pa.w = p / (rho * rho);
     ((V4<T>*)D.posa)[slot] = pa;
     ((T*)D.s_vel)[4 * slot + 3] = K.mass / rho;
+}
+
+template <typename T, int SM, int G>
+__global__ __launch_bounds__(512) void k_density(DevState D, int nblk, int remap)
+{
+    density_body<T, SM, G>(D, xcd_block(blockIdx.x, nblk, remap));
 }
 
 // ----------------------------------------------------------------- force
@@ -1487,9 +1494,8 @@ __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S
 // (xor shuffles), and lane 0 integrates, collides, erodes, stores the new
 // state and counts the particle's next cell.
 template <typename T, int SM, int G>
-__device__ __forceinline__ void force_body(const DevState& D, int nblk, int remap)
+__device__ __forceinline__ void force_body(const DevState& D, int b)
 {
-    const int b = xcd_block(blockIdx.x, nblk, remap);
     const int64_t gt = (int64_t)b * blockDim.x + threadIdx.x;
     const int64_t slot = gt / G;
     const int l = (int)(gt & (G - 1));
@@ -1574,13 +1580,85 @@ template <int SM, int G>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_force_f32(DevState D, int nblk,
                                                                                               int remap)
 {
-    force_body<float, SM, G>(D, nblk, remap);
+    force_body<float, SM, G>(D, xcd_block(blockIdx.x, nblk, remap));
 }
 
 template <int G>
 __global__ __launch_bounds__(512) void k_force_f64(DevState D, int nblk, int remap)
 {
-    force_body<double, SUM_PLAIN, G>(D, nblk, remap);
+    force_body<double, SUM_PLAIN, G>(D, xcd_block(blockIdx.x, nblk, remap));
+}
+
+// ------------------------------------------------------ persistent passes
+
+// Persistent form of the density and force passes: a grid of resident
+// blocks takes the passes' blocks (work items, in the flattened slot order
+// of all scenes) from eight queues, one per XCD: block p starts on queue
+// p mod 8 (the XCD the hardware gives it), whose k-th item is the one the
+// ordinary grid runs as block 8 k + x, remapped in runs of `chunk` as
+// xcd_block does; a block whose queue is empty takes items from the other
+// queues.  A block takes SPHE_GRAB consecutive items of a queue with one
+// atomic and asks for the next ones while it works on them.  Every item is
+// taken exactly once, and a block leaves when all eight queues are empty,
+// so every wave exits.  The queues (work[0..7], in items) are cleared by
+// k_work_reset before each launch.  The items of one block share no LDS
+// between waves (each lane group has its own region), so one barrier per
+// grab suffices.
+#define SPHE_GRAB 4
+__device__ __forceinline__ int64_t next_grab(uint32_t* work, int x, int nblk)
+{
+    for (int t = 0; t < 8; ++t) {
+        const int q = (x + t) & 7;
+        if ((int64_t)__hip_atomic_load(&work[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) * 8 + q >= nblk)
+            continue;  // drained
+        const int64_t bid = (int64_t)atomicAdd(&work[q], (uint32_t)SPHE_GRAB) * 8 + q;
+        if (bid < nblk) return bid;
+    }
+    return -1;
+}
+
+__global__ void k_work_reset(uint32_t* work)
+{
+    if (threadIdx.x < 8) work[threadIdx.x] = 0u;
+}
+
+// g: the first item of a grab as an ordinary block index 8 k + q
+#define SPHE_PERSISTENT_LOOP(BODY)                                                   \
+    __shared__ int64_t s_grab[2];                                                    \
+    const int x = (int)(blockIdx.x & 7u);                                            \
+    if (threadIdx.x == 0) s_grab[0] = next_grab(work, x, nblk);                      \
+    __syncthreads();                                                                 \
+    int64_t g = s_grab[0];                                                           \
+    for (int k = 0; g >= 0; ++k) {                                                   \
+        const int64_t nxt = threadIdx.x == 0 ? next_grab(work, x, nblk) : -1;        \
+        for (int i = 0; i < SPHE_GRAB; ++i) {                                        \
+            const int64_t bid = g + 8 * i;                                           \
+            if (bid >= nblk) break;                                                  \
+            const int b = xcd_block((int)bid, nblk, chunk);                          \
+            BODY;                                                                    \
+        }                                                                            \
+        if (threadIdx.x == 0) s_grab[(k + 1) & 1] = nxt;                             \
+        __syncthreads();                                                             \
+        g = s_grab[(k + 1) & 1];                                                     \
+    }
+
+template <typename T, int SM, int G>
+__global__ __launch_bounds__(512) void k_density_pers(DevState D, int nblk, int chunk, uint32_t* work)
+{
+    SPHE_PERSISTENT_LOOP((density_body<T, SM, G>(D, b)))
+}
+
+template <int SM, int G>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_force_f32_pers(
+    DevState D, int nblk, int chunk, uint32_t* work)
+{
+    SPHE_PERSISTENT_LOOP((force_body<float, SM, G>(D, b)))
+}
+
+template <int G>
+__global__ __launch_bounds__(512) void k_force_f64_pers(DevState D, int nblk, int chunk, uint32_t* work)
+{
+    SPHE_PERSISTENT_LOOP((force_body<double, SUM_PLAIN, G>(D, b)))
 }
 
 // --------------------------------------------------------------- terrain
@@ -2047,6 +2125,40 @@ static void klaunch_lds(void (*kern)(P...), dim3 grid, dim3 block, size_t lds, h
     else hipLaunchKernelGGL(kern, grid, block, lds, st, args...);
 }
 
+// Blocks of `kern` (block threads, lds bytes of dynamic LDS) resident on
+// the current device at once: the grid of a persistent launch (a multiple
+// of the XCD count).
+template <typename... P>
+static int resident_grid(void (*kern)(P...), int block, size_t lds)
+{
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, int, size_t, int>, int> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const auto key = std::make_tuple((const void*)kern, block, lds, dev);
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)kern, block, lds) != hipSuccess) per = 0;
+    (void)hipGetLastError();
+    const int n = std::max(8, (std::max(cus, 1) * std::max(per, 1)) / 8 * 8);
+    cache[key] = n;
+    return n;
+}
+
+// A pass as a persistent launch: the queues cleared, then a resident grid
+// (at most one block per item).
+template <typename... P, typename... A>
+static void klaunch_pers(void (*kern)(P...), int nblk, int block, size_t lds, const DevState& D, hipStream_t st,
+                         hipEvent_t e0, hipEvent_t e1, A... args)
+{
+    const int grid = std::min(resident_grid(kern, block, lds), nblk);
+    klaunch(k_work_reset, 1, 64, st, e0, nullptr, D.work);
+    klaunch_lds(kern, grid, block, lds, st, nullptr, e1, args...);
+}
+
 void sphe_launch_hash(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     if (D.P == 0) return;
@@ -2094,11 +2206,19 @@ template <int G>
 static void launch_density_g(const DevState& D, int block, int remap, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     // one row table and one list staging ring per group, at most 64 KiB per
-    // block (halve the block until it fits)
+    // block with the persistent form's static word (halve the block until
+    // it fits)
     const int W = 2 * D.yz + 1, ring = 4 * G > 16 ? 64 : 32;
-    while (block > 64 && density_lds_bytes(W, block / G, ring) > 65536) block /= 2;
+    while (block > 64 && density_lds_bytes(W, block / G, ring) > 65536 - 256) block /= 2;
     const size_t lds = (size_t)density_lds_bytes(W, block / G, ring);
     const int nblk = (int)grid_of(D.P * G, block);
+    if (D.persistent) {
+#define CALL(T, SM) \
+    klaunch_pers(k_density_pers<T, SM, G>, nblk, block, lds, D, st, e0, e1, D, nblk, remap, D.work)
+        SPHE_PREC_SWITCH(D.precision, CALL);
+#undef CALL
+        return;
+    }
 #define CALL(T, SM) klaunch_lds(k_density<T, SM, G>, nblk, block, lds, st, e0, e1, D, nblk, remap)
     SPHE_PREC_SWITCH(D.precision, CALL);
 #undef CALL
@@ -2108,6 +2228,16 @@ template <int G>
 static void launch_force_g(const DevState& D, int block, int remap, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     const int nblk = (int)grid_of(D.P * G, block);
+    if (D.persistent) {
+        if (D.precision == 2) {
+            klaunch_pers(k_force_f64_pers<G>, nblk, block, 0, D, st, e0, e1, D, nblk, remap, D.work);
+            return;
+        }
+#define CALL(T, SM) klaunch_pers(k_force_f32_pers<SM, G>, nblk, block, 0, D, st, e0, e1, D, nblk, remap, D.work)
+        SPHE_PREC_SWITCH(D.precision, CALL);
+#undef CALL
+        return;
+    }
     if (D.precision == 2) {
         klaunch(k_force_f64<G>, nblk, block, st, e0, e1, D, nblk, remap);
         return;

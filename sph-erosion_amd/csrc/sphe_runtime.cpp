@@ -116,6 +116,7 @@ struct sphe_ctx {
     int64_t pool_chunks = 0;  // overflow chunks the last growth asked for (0: none yet)
     int lanes = 4;   // lanes per particle in the density and force passes
     int scan_log2 = SPHE_SCAN_LOG2_DEFAULT;  // cells per scan tile: 2^scan_log2
+    bool persistent = false;  // density and force as persistent launches over work queues
     int sub_log2 = 2;  // x slabs per cell: 2^sub_log2
     int yz = 2;        // cells are 1/yz of the x edge along y and z
 
@@ -541,6 +542,8 @@ int build_layout(sphe_ctx* c)
     DALLOC(D.ptop, 64);
     DALLOC(D.ovf, 64);
     DALLOC(D.status, 64);
+    DALLOC(D.work, 64);
+    D.persistent = c->persistent ? 1 : 0;
     DALLOC(c->d_mass_part, (size_t)sphe_mass_blocks() * 3 * 8);
     DALLOC(c->d_mass_out, 64);
     DALLOC(c->d_inspect, 64 * 8);
@@ -862,13 +865,13 @@ void launch_step(sphe_ctx* c)
 std::string plan_signature(const sphe_ctx* c)
 {
     // everything the captured kernels take by value (DevState) or as launch shape
-    char b[352];
-    snprintf(b, sizeof(b), "%d/%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%p/%d/%d/%d/%d/%lld/%d", c->precision, c->block,
+    char b[384];
+    snprintf(b, sizeof(b), "%d/%d/%d/%d/%d/%lld/%lld/%lld/%d/%d/%p/%p/%p/%d/%d/%d/%d/%lld/%d/%d", c->precision, c->block,
              force_block(c), c->remap,
              c->lanes, (long long)c->D.P, (long long)c->D.C, (long long)c->D.TC, (int)any_erosion(c, -1),
              (int)any_erosion(c, SPHE_EROSION_TWO_PHASE), (const void*)c->D.u_pos, (const void*)c->D.nbl,
              (const void*)c->D.pool, c->D.maxn, c->D.sub_log2, c->D.yz, c->D.ocap, (long long)c->D.pool_cap,
-             c->D.scan_log2);
+             c->D.scan_log2, c->D.persistent);
     return b;
 }
 
@@ -2132,6 +2135,21 @@ int sphe_set_scan_tile(sphe_ctx* c, int log2_cells)
     HIPCHK(c, hipMemsetAsync(c->D.tfill, 0xFF, (size_t)c->D.npartial * 4, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->D.scan_log2 = log2_cells;
+    return SPHE_OK;
+}
+
+int sphe_set_persistent(sphe_ctx* c, int on)
+{
+    if (!c) return SPHE_EARG;
+    c->persistent = on != 0;
+    if (c->layout_valid) c->D.persistent = c->persistent ? 1 : 0;
+    return SPHE_OK;
+}
+
+int sphe_get_persistent(const sphe_ctx* c, int* on)
+{
+    if (!c || !on) return SPHE_EARG;
+    *on = c->persistent ? 1 : 0;
     return SPHE_OK;
 }
 

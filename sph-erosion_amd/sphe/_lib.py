@@ -103,6 +103,8 @@ SIGNATURES = {
     "sphe_autotune": (I, [P, I]),
     "sphe_autotune_grid": (I, [P, I]),
     "sphe_set_scan_tile": (I, [P, I]),
+    "sphe_set_persistent": (I, [P, I]),
+    "sphe_get_persistent": (I, [P, C.POINTER(I)]),
     "sphe_get_scan_tile": (I, [P, C.POINTER(I)]),
     "sphe_set_lanes": (I, [P, I]),
     "sphe_get_lanes": (I, [P, C.POINTER(I)]),

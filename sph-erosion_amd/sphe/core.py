@@ -370,6 +370,14 @@ class Context:
         self._chk(lib().sphe_autotune_grid(self._p, steps_per_trial), "autotune_grid")
         return self.get_yz_split(), self.get_subcells(), self.get_scan_tile()
 
+    def set_persistent(self, on):
+        self._chk(lib().sphe_set_persistent(self._p, int(on)), "set_persistent")
+
+    def get_persistent(self):
+        v = C.c_int()
+        self._chk(lib().sphe_get_persistent(self._p, C.byref(v)), "get_persistent")
+        return bool(v.value)
+
     def set_scan_tile(self, log2_cells):
         self._chk(lib().sphe_set_scan_tile(self._p, log2_cells), "set_scan_tile")
 

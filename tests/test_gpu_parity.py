@@ -209,6 +209,37 @@ def test_batch_equals_single_scenes():
         assert np.array_equal(batch.terrain_volume_fx(sid), single.terrain_volume_fx(ss))
 
 
+@pytest.mark.parametrize("lanes,precision", [(1, sphe.FP32), (4, sphe.FP32), (8, sphe.FP32_NAIVE),
+                                             (4, sphe.FP64)])
+def test_persistent_passes_change_no_bits(lanes, precision):
+    """Density and force as persistent launches over the work queues give the
+    same bits as the ordinary grids, on a batch with an empty scene, atomic
+    and two-phase erosion, in graph sequences and direct launches."""
+    spec = [(3000, 0.3, 64, sphe.EROSION_ATOMIC), None, (700, 0.45, 128, sphe.EROSION_TWO_PHASE),
+            (5000, 0.2, 192, sphe.EROSION_ATOMIC), (64, 0.3, 64, sphe.EROSION_ATOMIC)]
+    runs = []
+    for on, graph in ((False, True), (True, True), (True, False)):
+        ctx = sphe.Context(precision)
+        ctx.set_lanes(lanes)
+        ctx.set_persistent(on)
+        ctx.use_graph(graph)
+        assert ctx.get_persistent() == on
+        sids = []
+        for k, sp in enumerate(spec):
+            if sp is None:
+                ctx.add_scene(sphe.default_params())
+                continue
+            n, frac, T, ero = sp
+            sids.append(ctx.add_synthetic(scenes.dam_break_scene(70 + k, n, frac, T, erosion=ero, steps=40)))
+        ctx.step(40)
+        assert ctx.status() & 3 == 0
+        runs.append([ctx.state_f64(s) + (ctx.sediment_fx(s), ctx.terrain_volume_fx(s)) for s in sids])
+        ctx.close()
+    for r in runs[1:]:
+        for a, b in zip(runs[0], r):
+            assert all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
 def test_save_restore_repeats_the_same_steps():
     """A restored state steps to the same bits, also after the lists grew."""
     sc = small_dam()
