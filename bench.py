@@ -52,8 +52,11 @@ COUNTERS = os.path.join(ROOT, "profiles", "counters.json")
 SETTLE = {"cfg1": 100, "cfg2": 400, "cfg3": 500, "cfg4": 600, "cfg5": 1000, "cfg5_share": 1000}
 
 
+T_START = time.perf_counter()
+
+
 def log(msg):
-    print(msg, file=sys.stderr, flush=True)
+    print(f"[{time.perf_counter() - T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def make_workload(name):
@@ -413,6 +416,8 @@ def main():
     settle = SETTLE.get(a.workload, 0) if a.settle is None else a.settle
     log(f"bench: {sphe.lib().sphe_version().decode()} workload {a.workload}")
     scs, desc, run_steps = make_workload(a.workload)
+    if a.settle is None:  # the window stays inside the workload's run
+        settle = max(0, min(settle, run_steps - a.warmup - a.steps))
     N = sum(s.n for s in scs)
 
     ctx = sphe.Context(sphe.FP32)
