@@ -946,19 +946,27 @@ int resize_lists(sphe_ctx* c, int maxn, int ocap, int64_t chunks)
         *out = p;
         return ok;
     };
+    // neither size to be had: the buffer is gone, so the particle state (in
+    // other buffers, intact) goes to the host and the device layout is
+    // dropped; the next call rebuilds it or reports the shortage
+    auto lost = [&](const char* what) {
+        (void)sync_to_host(c);
+        free_device(c);
+        return fail(c, SPHE_ENOMEM, "%s lost: no device memory (the state is kept on the host)", what);
+    };
     bool ok = true;
     if (maxn != c->D.maxn) {
         void* p = nullptr;
         ok = replace(c->D.nbl, (size_t)P * c->D.maxn * 4, (size_t)P * maxn * 4, &p);
-        if (!p) return fail(c, SPHE_ENOMEM, "neighbour lists lost: no device memory");
         c->D.nbl = (uint32_t*)p;
+        if (!p) return lost("neighbour lists");
         if (ok) c->D.maxn = maxn;
     }
     if (ok && pool != c->D.pool_cap) {
         void* q = nullptr;
         ok = replace(c->D.pool, (size_t)c->D.pool_cap * 4, (size_t)pool * 4, &q);
-        if (!q) return fail(c, SPHE_ENOMEM, "overflow pool lost: no device memory");
         c->D.pool = (uint32_t*)q;
+        if (!q) return lost("overflow pool");
         if (ok) c->D.pool_cap = pool;
     }
     if (ok) c->D.ocap = ocap;
@@ -2350,10 +2358,20 @@ int sphe_autotune_grid(sphe_ctx* c, int steps_per_trial)
         if (r) return r;
         const bool prof = c->profiling;
         const auto saved_prof = c->prof;
+        const Tuned orig = best;
         double best_ms = 1e30;
+        // after a failed trial the grid goes back to what it was (best effort)
         auto restore = [&](int rc) {
             c->prof = saved_prof;
             c->profiling = prof;
+            if (rc != SPHE_OK) {
+                c->host_valid = true;
+                (void)sphe_set_yz_split(c, orig.yz);
+                (void)sphe_set_subcells(c, orig.slabs);
+                c->scan_log2 = orig.scan_log2;
+                if (c->layout_valid) (void)sphe_set_scan_tile(c, orig.scan_log2);
+                c->device_valid = false;
+            }
             return rc;
         };
         // the cell grid: y/z split and x slabs (the canonical order changes
