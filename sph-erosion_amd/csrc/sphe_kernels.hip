@@ -1583,8 +1583,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     force_body<float, SM, G>(D, xcd_block(blockIdx.x, nblk, remap));
 }
 
+// FP64: at most 168 registers (3 waves per SIMD instead of 2 unbounded).
 template <int G>
-__global__ __launch_bounds__(512) void k_force_f64(DevState D, int nblk, int remap)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_force_f64(DevState D, int nblk,
+                                                                                              int remap)
 {
     force_body<double, SUM_PLAIN, G>(D, xcd_block(blockIdx.x, nblk, remap));
 }
