@@ -390,10 +390,10 @@ void orc_derive(const orc_scene* s, orc_consts* c)
         c->box_hi[a] = c->box_c[a] + c->box_e;
     }
     /* cells: h (1 + 2^-8) along x, 1/yz of that along y and z (build-defined) */
-    c->yz = s->yz == 2 ? 2 : 1;
+    c->yz = s->yz == 2 || s->yz == 3 ? s->yz : 1;
     c->cell = h * 1.00390625f; /* h (1 + 2^-8) */
     c->inv_cell = 1.0f / c->cell;
-    c->inv_yz = c->inv_cell * (float)c->yz; /* exact */
+    c->inv_yz = c->inv_cell * (float)c->yz; /* exact for 1 and 2 */
     c->yz_scale = 1.0f / (float)c->yz;
     for (int a = 0; a < 3; a++) {
         c->grid_org[a] = c->box_lo[a];

@@ -102,7 +102,7 @@ typedef struct {
     double kc, ke, kd, band;        /* capacity coefficient, erode/deposit rates, contact band */
     int32_t lanes;                  /* lanes per particle splitting the neighbour sums (1, 2, 4, 8) */
     int32_t slabs;                  /* x slabs per grid cell (1, 2, 4): order and row clipping */
-    int32_t yz;                     /* cells are 1/yz of the x edge along y and z (1 or 2; 0 = 1) */
+    int32_t yz;                     /* cells are 1/yz of the x edge along y and z (1, 2 or 3; 0 = 1) */
 } orc_scene;
 
 /* Derived per-scene constants, identical in meaning to the GPU's. */

@@ -92,7 +92,7 @@ struct DevState {
     // occupied cells (an empty cell's word may be stale: readers clamp the
     // offsets to the cell)
     uint32_t* subw; int32_t sub_log2;
-    // cells are 1/yz of the x edge along y and z (yz = 1 or 2): the stencil
+    // cells are 1/yz of the x edge along y and z (yz = 1, 2 or 3): the stencil
     // has (2 yz + 1)^2 rows
     int32_t yz;
     int64_t* vol; float* height;

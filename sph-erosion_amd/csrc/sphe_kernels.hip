@@ -235,7 +235,7 @@ __device__ __forceinline__ T row_gap(T f)
 
 // The stencil rows of a particle, clipped to what the smoothing sphere can
 // reach.  Cells are h (1 + 2^-8) along x and 1/yz of that along y and z
-// (yz = 1 or 2), so the rows are the (2 yz + 1)^2 (y, z) columns around the
+// (yz = 1, 2 or 3), so the rows are the (2 yz + 1)^2 (y, z) columns around the
 // particle, numbered r = W (dz + yz) + (dy + yz), W = 2 yz + 1 (dz outer, dy
 // inner).  In x-cell units the radius is h / cell < 1; a row whose y and z
 // gaps g satisfy gy^2 + gz^2 > 1 is skipped, and the others keep the x
@@ -266,21 +266,21 @@ template <> struct RowWidth<float> {
 template <typename T>
 struct RowProbe {
     T gx, scale;
-    // gaps, in x-cell units, to the rows one and two cells below (l1, l2) and
-    // above (h1, h2) along y and z
-    T yl1, yl2, yh1, yh2, zl1, zl2, zh1, zh2;
+    // gaps, in x-cell units, to the rows one, two and three cells below (l1,
+    // l2, l3) and above (h1, h2, h3) along y and z (the third with yz = 3)
+    T yl1, yl2, yh1, yh2, zl1, zl2, zh1, zh2, yl3, yh3, zl3, zh3;
     int kx, ky, kz, sl, smax, fmax, flo_min, fhi_max;
     uint32_t rbase;  // global index of the first cell of the particle's own row
 };
 
-// r / W for 0 <= r < W^2 and W = 3 or 5 (multiply and shift)
+// r / W for 0 <= r < W^2 and W = 3, 5 or 7 (multiply and shift)
 __host__ __device__ constexpr int row_div(int r, int W)
 {
-    return W == 3 ? (r * 11) >> 5 : (r * 13) >> 6;
+    return W == 3 ? (r * 11) >> 5 : (W == 5 ? (r * 13) >> 6 : (r * 37) >> 8);
 }
 constexpr bool row_div_exact()
 {
-    for (int W = 3; W <= 5; W += 2)
+    for (int W = 3; W <= 7; W += 2)
         for (int r = 0; r < W * W; ++r)
             if (row_div(r, W) != r / W) return false;
     return true;
@@ -310,6 +310,13 @@ __device__ __forceinline__ RowProbe<T> row_probe(const DevState& D, const SceneD
     P.zh1 = row_gap<T>((T(1) - fz) * q);
     P.zl2 = row_gap<T>((T(1) + fz) * q);
     P.zh2 = row_gap<T>((T(1) + (T(1) - fz)) * q);
+    P.yl3 = P.yh3 = P.zl3 = P.zh3 = T(0);
+    if (D.yz > 2) {  // uniform
+        P.yl3 = row_gap<T>((T(2) + fy) * q);
+        P.yh3 = row_gap<T>((T(2) + (T(1) - fy)) * q);
+        P.zl3 = row_gap<T>((T(2) + fz) * q);
+        P.zh3 = row_gap<T>((T(2) + (T(1) - fz)) * q);
+    }
     P.sl = D.sub_log2;
     P.smax = (1 << P.sl) - 1;
     P.scale = T(1 << P.sl);
@@ -320,12 +327,17 @@ __device__ __forceinline__ RowProbe<T> row_probe(const DevState& D, const SceneD
 }
 
 template <typename T>
-__device__ __forceinline__ T gap_of(int d, T l1, T l2, T h1, T h2)
+__device__ __forceinline__ T gap_of(int d, int yz, T l1, T l2, T l3, T h1, T h2, T h3)
 {
-    // selects of values at hand (no branches): -1 l1, 1 h1, -2 l2, 2 h2, 0
+    // selects of values at hand: -1 l1, 1 h1, -2 l2, 2 h2, 0, and with
+    // yz = 3 (a uniform branch) -3 l3, 3 h3
     const T one = d < 0 ? l1 : h1;
     const T two = d < 0 ? l2 : h2;
-    const T g = (d == 2 || d == -2) ? two : one;
+    T g = (d == 2 || d == -2) ? two : one;
+    if (yz > 2) {
+        const T three = d < 0 ? l3 : h3;
+        g = (d == 3 || d == -3) ? three : g;
+    }
     return d == 0 ? T(0) : g;
 }
 
@@ -340,8 +352,8 @@ __device__ __forceinline__ bool row_range(const DevState& D, const SceneDev& S, 
 {
     const int ny = S.dims[1], nz = S.dims[2];
     const int z = P.kz + dzc, y = P.ky + dyc;
-    const T mz = gap_of<T>(dzc, P.zl1, P.zl2, P.zh1, P.zh2);
-    const T my = gap_of<T>(dyc, P.yl1, P.yl2, P.yh1, P.yh2);
+    const T mz = gap_of<T>(dzc, D.yz, P.zl1, P.zl2, P.zl3, P.zh1, P.zh2, P.zh3);
+    const T my = gap_of<T>(dyc, D.yz, P.yl1, P.yl2, P.yl3, P.yh1, P.yh2, P.yh3);
     const T r2 = my * my + mz * mz;
     const bool on = (z >= 0) & (z < nz) & (y >= 0) & (y < ny) & (r2 <= T(1));
     const T w = RowWidth<T>::of(T(1) - (on ? r2 : T(0))) + T(SPHE_ROW_MARGIN);

@@ -231,7 +231,7 @@ void grid_dims(const sphe_params& p, int dims[3], int yz)
     for (int a = 0; a < 3; ++a) {
         const float lo = p.box_center[a] - p.box_half;
         const float hi = p.box_center[a] + p.box_half;
-        const float ia = a == 0 ? inv : inv * (float)yz;  // exact: yz is 1 or 2
+        const float ia = a == 0 ? inv : inv * (float)yz;  // exact for yz 1 and 2
         dims[a] = (int)std::floor((hi - lo) * ia) + 1;
     }
 }
@@ -2109,7 +2109,7 @@ int sphe_get_subcells(const sphe_ctx* c, int* slabs)
 int sphe_set_yz_split(sphe_ctx* c, int split)
 {
     if (!c) return SPHE_EARG;
-    if (split != 1 && split != 2) return fail(c, SPHE_EARG, "yz split must be 1 or 2");
+    if (split < 1 || split > 3) return fail(c, SPHE_EARG, "yz split must be 1, 2 or 3");
     if (split == c->yz) return SPHE_OK;
     for (const Scene& s : c->scenes) {
         int dims[3];
@@ -2376,7 +2376,7 @@ int sphe_autotune_grid(sphe_ctx* c, int steps_per_trial)
         };
         // the cell grid: y/z split and x slabs (the canonical order changes
         // with them; every combination is a tested configuration)
-        for (int yz : {2, 1}) {
+        for (int yz : {2, 1, 3}) {
             for (int slabs : {4, 2, 1}) {
                 c->host_valid = true;  // a new split rebuilds the layout from the staged state
                 r = sphe_set_yz_split(c, yz);

@@ -334,7 +334,7 @@ def test_slab_split_bit_exact_twin(slabs):
     assert np.array_equal(ctx.sediment_fx(sid), st["sed"])
 
 
-@pytest.mark.parametrize("yz", [1])
+@pytest.mark.parametrize("yz", [1, 3])
 def test_yz_split_bit_exact_twin(yz):
     """The y/z cell split sets the grid, the canonical order and the clipped
     rows; the twin with the same split reproduces the GPU steps bit for bit
