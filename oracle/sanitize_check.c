@@ -128,7 +128,7 @@ int main(void)
     for (int m = 0; m < 4; ++m)
         for (int lanes = 1; lanes <= 8; lanes *= 2)
             for (int slabs = 1; slabs <= 4; slabs *= 2)
-                for (int yz = 1; yz <= 2; ++yz) {
+                for (int yz = 1; yz <= 3; ++yz) {
                     const int r = check_grid(reals[m][0], reals[m][1], lanes, slabs, yz, 3);
                     if (r) {
                         fprintf(stderr, "grid step real %d comp %d lanes %d slabs %d yz %d failed\n", reals[m][0],

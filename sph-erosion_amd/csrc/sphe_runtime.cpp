@@ -2380,7 +2380,12 @@ int sphe_autotune_grid(sphe_ctx* c, int steps_per_trial)
             for (int slabs : {4, 2, 1}) {
                 c->host_valid = true;  // a new split rebuilds the layout from the staged state
                 r = sphe_set_yz_split(c, yz);
+                if (r == SPHE_ERANGE) break;  // too many cells for this split: not a candidate
                 if (r == SPHE_OK) r = prepare(c);
+                if (r == SPHE_ENOMEM) {       // nor one whose grid does not fit
+                    (void)hipGetLastError();
+                    break;
+                }
                 if (r == SPHE_OK) r = sphe_set_subcells(c, slabs);
                 double ms = 0.0;
                 if (r == SPHE_OK) r = grid_trial(c, steps_per_trial, &ms);
