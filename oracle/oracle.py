@@ -303,6 +303,13 @@ def orc_step(scene, pos, vel, sed, ids, vol=None, height=None, real=8, comp=1, d
     vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(n, 3).copy()
     sed = np.ascontiguousarray(sed, dtype=np.int64).copy()
     ids = np.ascontiguousarray(ids, dtype=np.int32)
+    if len(sed) != n or len(ids) != n:
+        raise ValueError("sed and ids need one entry per particle")
+    if scene.tnx > 0 and scene.tnz > 0:
+        # the C step reads and writes tnx * tnz volumes and heights
+        nt = scene.tnx * scene.tnz
+        if vol is None or height is None or np.size(vol) != nt or np.size(height) != nt:
+            raise ValueError(f"a scene with a {scene.tnx} x {scene.tnz} terrain needs {nt} volumes and heights")
     if vol is not None:
         vol = np.ascontiguousarray(vol, dtype=np.int64).copy()
         height = np.ascontiguousarray(height, dtype=np.float32).copy()

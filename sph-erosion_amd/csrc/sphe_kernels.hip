@@ -573,7 +573,54 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
     scan_reduce_tile<ITEMS>(in, n, partial, occ, (int)blockIdx.x, lds);
 }
 
-// start[i] = exclusive prefix of counts.  With zero_counts the counts are
+// Exclusive prefix of the n tile totals, in place, by one block of 1,024
+// threads: chunks of 16,384 totals, 16 consecutive ones per thread, one block
+// scan per chunk and the running total carried to the next chunk.  Linear in
+// the number of tiles; each scan-down block then reads its tile's prefix
+// with one load.
+__global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* __restrict__ partial, int64_t n)
+{
+    constexpr int ITEMS = 16;
+    __shared__ uint32_t lds[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (int64_t c0 = 0; c0 < n; c0 += 1024 * ITEMS) {
+        const int64_t base = c0 + (int64_t)threadIdx.x * ITEMS;
+        uint32_t v[ITEMS];
+        load_run<ITEMS>(partial, base, n, v);
+        uint32_t s = 0;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) s += v[i];
+        uint32_t x = s;  // inclusive scan over the wave, then the 16 wave totals
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) lds[wave] = x;
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t t = lds[k];
+            if (k < wave) before += t;
+            tot += t;
+        }
+        uint32_t run = carry + before + (x - s);
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const uint32_t c = v[i];
+            v[i] = run;
+            run += c;
+        }
+        store_run<ITEMS>(partial, base, n, v);
+        carry += tot;
+        __syncthreads();  // the wave totals of the next chunk reuse lds
+    }
+}
+
+// start[i] = exclusive prefix of counts, from partial[b]: the total of the
+// tiles before tile b (k_scan_tiles).  With zero_counts the counts are
 // reset to zero for the next arrivals (the particle sort instead has each
 // particle's scatter reset its own cell); the optional single word zero is
 // reset too.  occ (optional): the tile flags; an unflagged tile's counts are
@@ -599,23 +646,13 @@ __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, u
     } else {
         load_run<ITEMS>(cnt, base, n, v);
     }
-    // the totals of the tiles before this one, eight loads in flight per thread
-    uint32_t prior = 0;
-    for (uint32_t q0 = threadIdx.x; q0 < blockIdx.x; q0 += 8 * blockDim.x) {
-        uint32_t pv[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t q = q0 + (uint32_t)k * blockDim.x;
-            pv[k] = q < blockIdx.x ? partial[q] : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) prior += pv[k];
-    }
+    // the total of the tiles before this one (k_scan_tiles' prefix)
+    const uint32_t ptot = partial[blockIdx.x];
     uint32_t s = 0;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) s += v[i];
-    uint32_t tot, ptot;
-    uint32_t run = block_scan_sum(s, prior, &tot, &ptot, lds);
+    uint32_t tot, none;
+    uint32_t run = block_scan_sum(s, 0u, &tot, &none, lds);
     run += ptot;
     uint32_t z[ITEMS];
 #pragma unroll
@@ -2188,6 +2225,7 @@ static void launch_scan_t(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* p
     const int64_t nb = (n + 256 * ITEMS - 1) / (256 * ITEMS);
     klaunch(k_scan_reduce<ITEMS>, (unsigned)nb, 256, st, e0, nullptr, (const uint32_t*)cnt, n, partial,
             (const uint32_t*)occ);
+    klaunch(k_scan_tiles, 1, 1024, st, nullptr, nullptr, partial, nb);
     klaunch(k_scan_down<ITEMS>, (unsigned)nb, 256, st, nullptr, e1, cnt, start, n, (const uint32_t*)partial,
             zero_counts ? 1 : 0, zero, occ, occ ? fill : nullptr);
 }
@@ -2292,6 +2330,7 @@ static void launch_scan_heights_t(const DevState& D, hipStream_t st, hipEvent_t 
     const int64_t nb = (n + 256 * ITEMS - 1) / (256 * ITEMS);
     const int64_t nh = (int64_t)grid_of(D.TC / 4, 256);  // TC: a multiple of 64
     klaunch(k_scan_reduce_heights<ITEMS>, (unsigned)(nb + nh), 256, st, e0, nullptr, D, (int)nb);
+    klaunch(k_scan_tiles, 1, 1024, st, nullptr, nullptr, D.partial, nb);
     klaunch(k_scan_down<ITEMS>, (unsigned)nb, 256, st, nullptr, e1, D.cnt, D.start, n, (const uint32_t*)D.partial,
             0, D.ptop, D.tocc, D.tfill);
 }

@@ -352,7 +352,7 @@ class Context:
         return v.value
 
     def set_yz_split(self, split):
-        """Cells 1/split of the x edge along y and z (1 or 2)."""
+        """Cells 1/split of the x edge along y and z (1, 2 or 3)."""
         self._chk(lib().sphe_set_yz_split(self._p, split), "set_yz_split")
 
     def get_yz_split(self):

@@ -366,7 +366,7 @@ def test_autotuned_grid_steps_bit_exact_with_the_twin():
     ctx, sid = gpu_context(sc)
     pos, vel, sed, vol, height, fx = oracle_inputs(ctx, sid, sc)
     yz, slabs, tile = ctx.autotune_grid(2)
-    assert yz in (1, 2) and slabs in (1, 2, 4) and 11 <= tile <= 13
+    assert yz in (1, 2, 3) and slabs in (1, 2, 4) and 11 <= tile <= 13
     gp0, gv0 = ctx.state_f64(sid)
     assert np.array_equal(gp0, pos) and np.array_equal(gv0, vel)
     assert ctx.autotune_grid(2) == (yz, slabs, tile)
@@ -380,6 +380,39 @@ def test_autotuned_grid_steps_bit_exact_with_the_twin():
         st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], sc.ids, st["vol"], st["height"], real=4, comp=1)
     assert np.array_equal(gp, st["pos"]) and np.array_equal(gv, st["vel"])
     assert np.array_equal(ctx.sediment_fx(sid), st["sed"])
+
+
+def test_grid_tuner_skips_a_split_with_too_many_cells():
+    """A box 402 cells long in x: split 3 would need 402 x 1204^2 = 5.8e8
+    cells, past the 4e8 a scene may hold, so setting it fails and leaves the
+    grid as it was, and the grid tuner leaves it out instead of failing.  Its
+    choice among splits 1 and 2 (up to 2.6e8 cells) steps bit-exact with the
+    twin configured the same way."""
+    base = small_dam(n=1024)
+    prm = base.params
+    prm.box_half = 9.2
+    sc = scenes.Scene("big_box_1024", prm, None, base.pos, base.vel, base.ids, 2)
+    ctx, sid = gpu_context(sc)
+    split0 = ctx.get_yz_split()
+    with pytest.raises(sphe.SpheError, match="ERANGE"):
+        ctx.set_yz_split(3)
+    assert ctx.get_yz_split() == split0
+    yz, slabs, tile = ctx.autotune_grid(1)
+    assert yz in (1, 2) and slabs in (1, 2, 4) and 11 <= tile <= 13
+    pos, vel, sed, _, _, _ = oracle_inputs(ctx, sid, sc)
+    gp0, gv0 = ctx.state_f64(sid)
+    assert np.array_equal(gp0, pos) and np.array_equal(gv0, vel)
+    osc = orc_scene_from(sc.params)
+    osc.yz = yz
+    osc.slabs = slabs
+    ctx.step(2)
+    assert ctx.status() & 3 == 0
+    gp, gv = ctx.state_f64(sid)
+    st = dict(pos=pos, vel=vel, sed=sed)
+    for _ in range(2):
+        st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], sc.ids, real=4, comp=1)
+    assert np.array_equal(gp, st["pos"]) and np.array_equal(gv, st["vel"])
+    ctx.close()
 
 
 def test_scan_tile_never_changes_results():
