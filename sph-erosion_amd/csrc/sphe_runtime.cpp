@@ -2380,9 +2380,10 @@ int sphe_autotune_grid(sphe_ctx* c, int steps_per_trial)
             for (int slabs : {4, 2, 1}) {
                 c->host_valid = true;  // a new split rebuilds the layout from the staged state
                 r = sphe_set_yz_split(c, yz);
-                if (r == SPHE_ERANGE) break;  // too many cells for this split: not a candidate
                 if (r == SPHE_OK) r = prepare(c);
-                if (r == SPHE_ENOMEM) {       // nor one whose grid does not fit
+                if (r == SPHE_ERANGE || r == SPHE_ENOMEM) {
+                    // a split with too many cells, or whose grid does not fit
+                    // in memory, is not a candidate (the grid is rebuilt below)
                     (void)hipGetLastError();
                     break;
                 }
