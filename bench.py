@@ -26,9 +26,10 @@ Prints one JSON line:
     exact fixed-point units);
   * density error against the library's FP64 path on the final state; the
     accuracy protocol (one step in each FP32 mode and FP64 from a shared
-    state); the plain FP64 GPU path over the same window; and the reference
-    step on the host's cores (the CPU oracle's restatement of it, the only
-    use of the oracle here).
+    state); the front end's frame loop (a step, then the largest scene's
+    positions read back to the host); the plain FP64 GPU path over the same
+    window; and the reference step on the host's cores (the CPU oracle's
+    restatement of it, the only use of the oracle here).
 Exits with status 3 when the step reported a non-finite state or a particle it
 could not place.
 """
@@ -144,6 +145,37 @@ def density_error(ctx, sids, scs, end_state, tuning):
     ref = np.concatenate([c64.debug_density(s) for s in s64])
     c64.close()
     return float((np.abs(rho - ref) / ref).max())
+
+
+def front_end_loop(ctx, sids, scs, steps):
+    """The drop-in front end's frame loop (main.cpp:313 Run, then drawing):
+    one step, then the positions of the largest scene read back into a host
+    buffer the caller keeps (12 bytes per particle over PCIe), `steps` times.
+    Not the headline rate: it includes the read-back and a synchronisation
+    per step."""
+    import ctypes as C
+
+    import sphe
+    big = max(range(len(sids)), key=lambda i: scs[i].n)
+    n_big = scs[big].n
+    buf = np.zeros((n_big, 3), dtype=np.float32)
+    ptr = buf.ctypes.data_as(C.c_void_p)
+
+    def read():
+        rc = sphe.lib().sphe_read_buffer(ctx._p, sids[big], sphe.BUF_POS, ptr, buf.nbytes)
+        if rc != 0:
+            raise RuntimeError(f"sphe_read_buffer: {rc}")
+
+    read()  # the host buffer's pages are touched once before the clock starts
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        ctx.step(1)
+        read()
+    el = time.perf_counter() - t
+    n = sum(s.n for s in scs)
+    return dict(steps=steps, readback_particles=n_big, readback_bytes_per_step=12 * n_big,
+                ms_per_step=1e3 * el / steps, particle_steps_per_s=n * steps / el)
 
 
 def accuracy_protocol(scs, start_state, tuning, warm=50):
@@ -551,6 +583,7 @@ def main():
         out["density_rel_err_max"] = density_error(ctx, sids, scs, end_state, tuning)
         log("bench: accuracy protocol")
         out["accuracy"] = accuracy_protocol(scs, start_state, tuning)
+        out["front_end_loop"] = front_end_loop(ctx, sids, scs, 10)
 
     if not a.no_fp64:
         log("bench: FP64 plain GPU path over the same window")

@@ -157,7 +157,11 @@ int sphe_set_particles(sphe_ctx* ctx, int scene, int n, const float* pos3, const
 int sphe_set_particles_f64(sphe_ctx* ctx, int scene, int n, const double* pos3, const double* vel3,
                            const int32_t* ids, const int64_t* sed_fx);
 int sphe_num_particles(const sphe_ctx* ctx, int scene, int* n);
-/* Copies one per-particle buffer (SPHE_BUF_*) in ascending id order. */
+/* Copies one per-particle buffer (SPHE_BUF_*) in ascending id order.  After
+ * a step, positions and velocities of a scene whose ids are 0 .. n-1 are
+ * gathered in id order on the device and copied alone (12 bytes per
+ * particle); other reads bring the whole context's state to the host once
+ * per step. */
 int sphe_read_buffer(sphe_ctx* ctx, int scene, int which, void* out, size_t bytes);
 /* The same buffer as a host view owned by the context: *view points at
  * *count records (3 floats for POS and VEL, else 1 float or int32), a
@@ -222,6 +226,15 @@ int sphe_terrain_mesh(sphe_ctx* ctx, int scene, float* verts, size_t* nverts_flo
  * the host round trip of Grid::UpdateGrid + glBufferData
  * (grid.h:138-176, main.cpp:228-240). */
 int sphe_terrain_mesh_device(sphe_ctx* ctx, int scene, const float** dverts, size_t* nfloats);
+/* The scene's particle positions, 3 floats each in ascending id order, left
+ * in device memory for a renderer that copies them into an instance buffer
+ * through graphics interop and draws every particle in one instanced call.
+ * Replaces FluidSystemSPH::Draw's one draw call per particle
+ * (fluid_system.h:185-204).  The scene's ids must be 0 .. n-1 (Initialize,
+ * AddParticles and Reset give them; SPHE_EARG otherwise).  *dpos stays valid
+ * until the next call that steps, edits or destroys the context, or reads a
+ * gathered buffer again. */
+int sphe_positions_device(sphe_ctx* ctx, int scene, const float** dpos, size_t* nfloats);
 
 /* ---- stepping ---------------------------------------------------------- */
 

@@ -51,6 +51,7 @@ struct Scene {
     int64_t p_begin = 0, p_cap = 0, cell_off = 0, ncell = 0, t_off = 0, tcap = 0;
     int dims[3] = {0, 0, 0};
     int64_t idmap_off = -1, idmap_len = 0;  // the scene's part of the id -> slot map (-1: none)
+    bool ids_dense = false;                 // the ids are 0 .. n-1 (the device gathers in id order)
 };
 
 struct ProfRec {
@@ -142,6 +143,8 @@ struct sphe_ctx {
     int step_sub_log2 = 2;
     float* d_mesh = nullptr;  // terrain vertex stream (sphe_terrain_mesh*)
     size_t mesh_cap = 0;      // floats
+    float* d_feed = nullptr;  // one scene's positions or velocities in id order
+    size_t feed_cap = 0;      // floats
 
     // sphe_save_state / sphe_restore_state: particle and terrain state of
     // every scene, in the host staging layout
@@ -476,6 +479,8 @@ int build_layout(sphe_ctx* c)
     for (Scene& s : c->scenes) {
         s.idmap_off = -1;
         s.idmap_len = 0;
+        s.ids_dense = true;
+        for (size_t i = 0; i < s.ids.size() && s.ids_dense; ++i) s.ids_dense = s.ids[i] == (int32_t)i;
         if (s.ids.empty()) continue;
         const int64_t len = (int64_t)s.ids.back() + 1;
         if (len > 2 * (int64_t)s.ids.size() + 4096 || M + len > INT32_MAX / 2) continue;
@@ -1254,6 +1259,7 @@ int sphe_destroy(sphe_ctx* c)
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     free_device(c);
     if (c->d_mesh) (void)hipFree(c->d_mesh);
+    if (c->d_feed) (void)hipFree(c->d_feed);
     if (c->d_mass_all) (void)hipFree(c->d_mass_all);
     for (hipEvent_t e : c->seq_ev)
         if (e) (void)hipEventDestroy(e);
@@ -1477,14 +1483,47 @@ static int read_rho(sphe_ctx* c, Scene& s, std::vector<double>& rho)
     return SPHE_OK;
 }
 
+// Positions (vel false) or velocities of a scene whose ids are 0 .. n-1, as
+// 3 floats per particle in id order, gathered on the device from the current
+// state into c->d_feed.
+static int feed_on_device(sphe_ctx* c, const Scene* s, bool vel, float** out)
+{
+    const size_t nf = 3 * s->ids.size();
+    int r = ensure_device(c);
+    if (r) return r;
+    if (nf > c->feed_cap) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->d_feed) (void)hipFree(c->d_feed);
+        c->d_feed = nullptr;
+        c->feed_cap = 0;
+        HIPCHK(c, try_malloc(&c->d_feed, nf * 4));
+        c->feed_cap = nf;
+    }
+    sphe_launch_feed(c->D, s->p_begin, (int64_t)s->ids.size(), vel, c->d_feed, c->stream);
+    HIPCHK(c, hipGetLastError());
+    *out = c->d_feed;
+    return SPHE_OK;
+}
+
 int sphe_read_buffer(sphe_ctx* c, int scene, int which, void* out, size_t bytes)
 {
     SCENE_OR_FAIL(s, c, scene);
-    int r = sync_to_host(c);
-    if (r) return r;
     const size_t n = s->ids.size();
     const size_t comps = (which == SPHE_BUF_POS || which == SPHE_BUF_VEL) ? 3 : 1;
     if (!out || bytes < n * comps * 4) return fail(c, SPHE_EARG, "output buffer too small");
+    if ((which == SPHE_BUF_POS || which == SPHE_BUF_VEL) && n > 0 && !c->host_valid && c->device_valid &&
+        s->ids_dense) {
+        // the device holds the newer state: gather this scene alone in id
+        // order and copy 12 bytes per particle, with no host sync of the rest
+        float* d = nullptr;
+        int r = feed_on_device(c, s, which == SPHE_BUF_VEL, &d);
+        if (r) return r;
+        HIPCHK(c, hipMemcpyAsync(out, d, n * 12, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return SPHE_OK;
+    }
+    int r = sync_to_host(c);
+    if (r) return r;
     float* f = (float*)out;
     switch (which) {
     case SPHE_BUF_POS:
@@ -1800,6 +1839,26 @@ int sphe_terrain_mesh_device(sphe_ctx* c, int scene, const float** dverts, size_
     int r = mesh_on_device(c, s, &d);
     if (r) return r;
     *dverts = d;
+    return SPHE_OK;
+}
+
+int sphe_positions_device(sphe_ctx* c, int scene, const float** dpos, size_t* nfloats)
+{
+    SCENE_OR_FAIL(s, c, scene);
+    if (!dpos || !nfloats) return fail(c, SPHE_EARG, "null output");
+    *dpos = nullptr;
+    *nfloats = 3 * s->ids.size();
+    if (s->ids.empty()) return SPHE_OK;
+    if (!c->layout_valid) {
+        int r = build_layout(c);  // decides whether the ids are 0 .. n-1
+        if (r) return r;
+    }
+    if (!s->ids_dense) return fail(c, SPHE_EARG, "the scene's ids are not 0 .. n-1");
+    float* d = nullptr;
+    int r = feed_on_device(c, s, false, &d);
+    if (r) return r;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *dpos = d;
     return SPHE_OK;
 }
 

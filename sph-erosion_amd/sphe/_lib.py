@@ -85,6 +85,7 @@ SIGNATURES = {
     "sphe_fixed_exponent": (I, [P, I, C.POINTER(I)]),
     "sphe_terrain_mesh": (I, [P, I, P, C.POINTER(C.c_size_t), P, C.POINTER(C.c_size_t)]),
     "sphe_terrain_mesh_device": (I, [P, I, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
+    "sphe_positions_device": (I, [P, I, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
     "sphe_step": (I, [P, I]),
     "sphe_sync": (I, [P]),
     "sphe_use_graph": (I, [P, I]),

@@ -90,6 +90,50 @@ def test_terrain_mesh_after_erosion_matches_oracle():
     assert np.array_equal(_hip_copy(dptr.value, n.value * 4).view(np.uint32), ref.view(np.uint32))
 
 
+@pytest.mark.parametrize("precision", [sphe.FP32, sphe.FP64])
+def test_particle_feed_gathered_on_the_device(precision):
+    """After steps, positions and velocities read back for a scene with ids
+    0 .. n-1 (gathered in id order on the device, without a host sync) are
+    the FP32 roundings of the state the host sync returns, bit for bit, and
+    sphe_positions_device holds the same positions.  A scene with sparse ids
+    reads through the host with the same result and has no device feed.
+    Reading changes nothing in the steps that follow."""
+    import ctypes as C
+
+    def build():
+        sc = scenes.cfg1(n=4096)
+        sc2 = scenes.cfg1(seed_idx=3, n=1500)
+        ctx = sphe.Context(precision)
+        sid = ctx.add_synthetic(sc)
+        sid2 = ctx.add_scene(sc2.params, sc2.terrain)
+        ctx.set_particles(sid2, sc2.pos, sc2.vel, ids=3 * np.arange(sc2.n, dtype=np.int32))
+        return ctx, sid, sid2
+
+    ctx, sid, sid2 = build()
+    ctx.step(6)
+    p, v, p2 = ctx.read(sid, sphe.BUF_POS), ctx.read(sid, sphe.BUF_VEL), ctx.read(sid2, sphe.BUF_POS)
+    dptr, n = C.c_void_p(), C.c_size_t()
+    assert sphe.lib().sphe_positions_device(ctx._p, sid, C.byref(dptr), C.byref(n)) == 0
+    assert n.value == 3 * 4096 and dptr.value
+    dev = _hip_copy(dptr.value, n.value * 4).reshape(-1, 3)
+    assert sphe.lib().sphe_positions_device(ctx._p, sid2, C.byref(dptr), C.byref(n)) != 0
+    ctx.step(3)
+    end = ctx.state_f64(sid)
+
+    ref, sid_r, sid2_r = build()
+    ref.step(6)
+    gp, gv = ref.state_f64(sid_r)  # the host sync of every scene
+    gp2, _ = ref.state_f64(sid2_r)
+    for got, want in ((p, gp), (v, gv), (dev, gp), (p2, gp2)):
+        assert np.array_equal(got.view(np.uint32), want.astype(np.float32).view(np.uint32))
+    assert np.array_equal(ref.read(sid_r, sphe.BUF_POS), p)  # the host path, same bits
+    ref.step(3)
+    ref_end = ref.state_f64(sid_r)
+    assert np.array_equal(end[0], ref_end[0]) and np.array_equal(end[1], ref_end[1])
+    ctx.close()
+    ref.close()
+
+
 def test_buffer_views_and_batches():
     """sphe_buffer's host views equal sphe_read_buffer's copies; a batch made
     by sphe_batch_create (terrains included) and stepped by sphe_batch_step
