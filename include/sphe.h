@@ -158,10 +158,10 @@ int sphe_set_particles_f64(sphe_ctx* ctx, int scene, int n, const double* pos3, 
                            const int32_t* ids, const int64_t* sed_fx);
 int sphe_num_particles(const sphe_ctx* ctx, int scene, int* n);
 /* Copies one per-particle buffer (SPHE_BUF_*) in ascending id order.  After
- * a step, positions and velocities of a scene whose ids are 0 .. n-1 are
- * gathered in id order on the device and copied alone (12 bytes per
- * particle); other reads bring the whole context's state to the host once
- * per step. */
+ * a step, positions, velocities, densities and pressures of a scene whose
+ * ids are 0 .. n-1 are gathered in id order on the device and copied alone;
+ * positions, velocities and sediment of other scenes bring the whole
+ * context's state to the host once per step. */
 int sphe_read_buffer(sphe_ctx* ctx, int scene, int which, void* out, size_t bytes);
 /* The same buffer as a host view owned by the context: *view points at
  * *count records (3 floats for POS and VEL, else 1 float or int32), a

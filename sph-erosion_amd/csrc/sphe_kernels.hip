@@ -1868,19 +1868,23 @@ __global__ __launch_bounds__(256) void k_idmap(DevState D, int32_t* __restrict__
     if (id >= 0 && id < S.idmap_len) atomicMin(&map[S.idmap_off + id], (int32_t)slot);
 }
 
-// Positions or velocities of the n particles at slots p0 .. p0+n-1 in id
-// order (the ids are 0 .. n-1): the current state lies at the last step's
-// sorted slots with its ids in u_id.  Each coordinate is rounded to FP32 once,
-// as the host copy rounds it.
+// Positions (what 0), velocities (1) or the last step's densities (2) of the
+// n particles at slots p0 .. p0+n-1 in id order (the ids are 0 .. n-1): the
+// current state lies at the last step's sorted slots with its ids in u_id.
+// Each value is rounded to FP32 once, as the host copy rounds it.
 template <typename T>
-__global__ __launch_bounds__(256) void k_feed(DevState D, int64_t p0, int64_t n, int vel, float* __restrict__ out)
+__global__ __launch_bounds__(256) void k_feed(DevState D, int64_t p0, int64_t n, int what, float* __restrict__ out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t slot = p0 + i;
     const int32_t id = D.u_id[slot];
     if (id < 0 || (int64_t)id >= n) return;
-    const V4<T> v = ((const V4<T>*)(vel ? D.u_vel : D.u_pos))[slot];
+    if (what == 2) {
+        out[id] = (float)((const T*)D.rho)[slot];
+        return;
+    }
+    const V4<T> v = ((const V4<T>*)(what ? D.u_vel : D.u_pos))[slot];
     float* o = out + 3 * (int64_t)id;
     o[0] = (float)v.x;
     o[1] = (float)v.y;
@@ -2416,11 +2420,11 @@ void sphe_launch_idmap(const DevState& D, int32_t* map, int64_t len, hipStream_t
     if (D.P > 0 && len > 0) k_idmap<<<grid_of(D.P, 256), 256, 0, st>>>(D, map);
 }
 
-void sphe_launch_feed(const DevState& D, int64_t p0, int64_t n, bool vel, float* out, hipStream_t st)
+void sphe_launch_feed(const DevState& D, int64_t p0, int64_t n, int what, float* out, hipStream_t st)
 {
     if (n <= 0) return;
-    if (D.precision == 2) k_feed<double><<<grid_of(n, 256), 256, 0, st>>>(D, p0, n, vel ? 1 : 0, out);
-    else k_feed<float><<<grid_of(n, 256), 256, 0, st>>>(D, p0, n, vel ? 1 : 0, out);
+    if (D.precision == 2) k_feed<double><<<grid_of(n, 256), 256, 0, st>>>(D, p0, n, what, out);
+    else k_feed<float><<<grid_of(n, 256), 256, 0, st>>>(D, p0, n, what, out);
 }
 
 void sphe_launch_find_id(const DevState& D, int scene, int32_t id, int32_t* slot_out, hipStream_t st)

@@ -36,9 +36,10 @@ void sphe_launch_terrain_mesh(const float* H, int nx, int nz, float cell, float 
 void sphe_launch_inspect(const DevState& D, const int32_t* slotp, int lanes, double* out, hipStream_t st);
 // id -> sorted slot map of the last step for the scenes that have one.
 void sphe_launch_idmap(const DevState& D, int32_t* map, int64_t len, hipStream_t st);
-// Positions (vel false) or velocities of the n particles at slots p0 ..
-// p0+n-1, whose ids are 0 .. n-1, as 3 floats at out[3 id].
-void sphe_launch_feed(const DevState& D, int64_t p0, int64_t n, bool vel, float* out, hipStream_t st);
+// Positions (what 0) or velocities (1) of the n particles at slots p0 ..
+// p0+n-1, whose ids are 0 .. n-1, as 3 floats at out[3 id]; their last
+// densities (2) as one float at out[id].
+void sphe_launch_feed(const DevState& D, int64_t p0, int64_t n, int what, float* out, hipStream_t st);
 // Sorted slot of one particle id of a scene by a search over its slots.
 void sphe_launch_find_id(const DevState& D, int scene, int32_t id, int32_t* slot_out, hipStream_t st);
 void sphe_launch_accel(const DevState& D, int scene, int lanes, double* out, hipStream_t st);

@@ -1483,12 +1483,14 @@ static int read_rho(sphe_ctx* c, Scene& s, std::vector<double>& rho)
     return SPHE_OK;
 }
 
-// Positions (vel false) or velocities of a scene whose ids are 0 .. n-1, as
-// 3 floats per particle in id order, gathered on the device from the current
-// state into c->d_feed.
-static int feed_on_device(sphe_ctx* c, const Scene* s, bool vel, float** out)
+enum { FEED_POS = 0, FEED_VEL = 1, FEED_RHO = 2 };
+
+// Positions or velocities (3 floats per particle) or the last step's
+// densities (1 float) of a scene whose ids are 0 .. n-1, in id order,
+// gathered on the device from the current state into c->d_feed.
+static int feed_on_device(sphe_ctx* c, const Scene* s, int what, float** out)
 {
-    const size_t nf = 3 * s->ids.size();
+    const size_t nf = (what == FEED_RHO ? 1 : 3) * s->ids.size();
     int r = ensure_device(c);
     if (r) return r;
     if (nf > c->feed_cap) {
@@ -1499,7 +1501,7 @@ static int feed_on_device(sphe_ctx* c, const Scene* s, bool vel, float** out)
         HIPCHK(c, try_malloc(&c->d_feed, nf * 4));
         c->feed_cap = nf;
     }
-    sphe_launch_feed(c->D, s->p_begin, (int64_t)s->ids.size(), vel, c->d_feed, c->stream);
+    sphe_launch_feed(c->D, s->p_begin, (int64_t)s->ids.size(), what, c->d_feed, c->stream);
     HIPCHK(c, hipGetLastError());
     *out = c->d_feed;
     return SPHE_OK;
@@ -1511,20 +1513,26 @@ int sphe_read_buffer(sphe_ctx* c, int scene, int which, void* out, size_t bytes)
     const size_t n = s->ids.size();
     const size_t comps = (which == SPHE_BUF_POS || which == SPHE_BUF_VEL) ? 3 : 1;
     if (!out || bytes < n * comps * 4) return fail(c, SPHE_EARG, "output buffer too small");
-    if ((which == SPHE_BUF_POS || which == SPHE_BUF_VEL) && n > 0 && !c->host_valid && c->device_valid &&
-        s->ids_dense) {
-        // the device holds the newer state: gather this scene alone in id
-        // order and copy 12 bytes per particle, with no host sync of the rest
+    if (which < SPHE_BUF_POS || which > SPHE_BUF_ID) return fail(c, SPHE_EARG, "unknown buffer id %d", which);
+    const bool state = which == SPHE_BUF_POS || which == SPHE_BUF_VEL;
+    const bool rho = which == SPHE_BUF_DENSITY || which == SPHE_BUF_PRESS;
+    float* f = (float*)out;
+    if (n > 0 && s->ids_dense && c->device_valid && ((state && !c->host_valid) || (rho && c->stepped))) {
+        // the device holds the values: gather this scene alone in id order
+        // and copy 4 bytes per value, with no host sync of the rest
         float* d = nullptr;
-        int r = feed_on_device(c, s, which == SPHE_BUF_VEL, &d);
+        int r = feed_on_device(c, s, which == SPHE_BUF_POS ? FEED_POS : (state ? FEED_VEL : FEED_RHO), &d);
         if (r) return r;
-        HIPCHK(c, hipMemcpyAsync(out, d, n * 12, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(out, d, n * comps * 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (which == SPHE_BUF_PRESS)
+            for (size_t i = 0; i < n; ++i) f[i] = s->prm.stiffness * (f[i] - s->prm.rest_density);
         return SPHE_OK;
     }
-    int r = sync_to_host(c);
+    // the ids (ascending), the mass and the densities (from the device) need
+    // no host copy of the state
+    int r = state || which == SPHE_BUF_SED ? sync_to_host(c) : SPHE_OK;
     if (r) return r;
-    float* f = (float*)out;
     switch (which) {
     case SPHE_BUF_POS:
         for (size_t i = 0; i < 3 * n; ++i) f[i] = (float)s->pos[i];
@@ -1855,7 +1863,7 @@ int sphe_positions_device(sphe_ctx* c, int scene, const float** dpos, size_t* nf
     }
     if (!s->ids_dense) return fail(c, SPHE_EARG, "the scene's ids are not 0 .. n-1");
     float* d = nullptr;
-    int r = feed_on_device(c, s, false, &d);
+    int r = feed_on_device(c, s, FEED_POS, &d);
     if (r) return r;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *dpos = d;

@@ -92,46 +92,39 @@ def test_terrain_mesh_after_erosion_matches_oracle():
 
 @pytest.mark.parametrize("precision", [sphe.FP32, sphe.FP64])
 def test_particle_feed_gathered_on_the_device(precision):
-    """After steps, positions and velocities read back for a scene with ids
-    0 .. n-1 (gathered in id order on the device, without a host sync) are
-    the FP32 roundings of the state the host sync returns, bit for bit, and
-    sphe_positions_device holds the same positions.  A scene with sparse ids
-    reads through the host with the same result and has no device feed.
-    Reading changes nothing in the steps that follow."""
+    """Two scenes hold the same particles, one with ids 0 .. n-1 and one with
+    ids 2i (the same order inside every cell, so the two step to the same
+    bits).  After steps, positions, velocities, densities and pressures of the
+    first are gathered in id order on the device, with no host sync; the
+    second reads through the host; the two are equal bit for bit, positions
+    and velocities are the FP32 roundings of the synced state, and
+    sphe_positions_device holds the same positions (the second scene has
+    none).  Reading changes nothing in the steps that follow."""
     import ctypes as C
 
-    def build():
-        sc = scenes.cfg1(n=4096)
-        sc2 = scenes.cfg1(seed_idx=3, n=1500)
-        ctx = sphe.Context(precision)
-        sid = ctx.add_synthetic(sc)
-        sid2 = ctx.add_scene(sc2.params, sc2.terrain)
-        ctx.set_particles(sid2, sc2.pos, sc2.vel, ids=3 * np.arange(sc2.n, dtype=np.int32))
-        return ctx, sid, sid2
-
-    ctx, sid, sid2 = build()
+    sc = scenes.cfg1(n=4096)
+    ctx = sphe.Context(precision)
+    a = ctx.add_synthetic(sc)
+    b = ctx.add_scene(sc.params, sc.terrain)
+    ctx.set_particles(b, sc.pos, sc.vel, ids=2 * sc.ids)
     ctx.step(6)
-    p, v, p2 = ctx.read(sid, sphe.BUF_POS), ctx.read(sid, sphe.BUF_VEL), ctx.read(sid2, sphe.BUF_POS)
+    bufs = (sphe.BUF_POS, sphe.BUF_VEL, sphe.BUF_DENSITY, sphe.BUF_PRESS)
+    got = [ctx.read(a, w) for w in bufs]  # device gathers, before anything syncs the host
     dptr, n = C.c_void_p(), C.c_size_t()
-    assert sphe.lib().sphe_positions_device(ctx._p, sid, C.byref(dptr), C.byref(n)) == 0
-    assert n.value == 3 * 4096 and dptr.value
+    assert sphe.lib().sphe_positions_device(ctx._p, a, C.byref(dptr), C.byref(n)) == 0
+    assert n.value == 3 * sc.n and dptr.value
     dev = _hip_copy(dptr.value, n.value * 4).reshape(-1, 3)
-    assert sphe.lib().sphe_positions_device(ctx._p, sid2, C.byref(dptr), C.byref(n)) != 0
+    assert sphe.lib().sphe_positions_device(ctx._p, b, C.byref(dptr), C.byref(n)) != 0
+    want = [ctx.read(b, w) for w in bufs]  # the host path
+    assert np.all(got[2] > 0)
+    for g, w in zip(got + [dev], want + [want[0]]):
+        assert np.array_equal(g.view(np.uint32), w.view(np.uint32))
+    gp, gv = ctx.state_f64(a)
+    assert np.array_equal(got[0], gp.astype(np.float32)) and np.array_equal(got[1], gv.astype(np.float32))
     ctx.step(3)
-    end = ctx.state_f64(sid)
-
-    ref, sid_r, sid2_r = build()
-    ref.step(6)
-    gp, gv = ref.state_f64(sid_r)  # the host sync of every scene
-    gp2, _ = ref.state_f64(sid2_r)
-    for got, want in ((p, gp), (v, gv), (dev, gp), (p2, gp2)):
-        assert np.array_equal(got.view(np.uint32), want.astype(np.float32).view(np.uint32))
-    assert np.array_equal(ref.read(sid_r, sphe.BUF_POS), p)  # the host path, same bits
-    ref.step(3)
-    ref_end = ref.state_f64(sid_r)
-    assert np.array_equal(end[0], ref_end[0]) and np.array_equal(end[1], ref_end[1])
+    ea, eb = ctx.state_f64(a), ctx.state_f64(b)
+    assert np.array_equal(ea[0], eb[0]) and np.array_equal(ea[1], eb[1])
     ctx.close()
-    ref.close()
 
 
 def test_buffer_views_and_batches():
