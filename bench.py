@@ -589,10 +589,11 @@ def main():
         log("bench: FP64 plain GPU path over the same window")
         ctx.close()  # the FP64 context gets the device to itself
         c64, s64 = context_from(scs, start_state, sphe.FP64, tuning)
-        # one step sizes its neighbour lists to the state (as the settle did
-        # for the FP32 context), then back to the window's start
+        # the window run once untimed sizes its neighbour lists for every
+        # step of it (as the settle did for the FP32 context), then back to
+        # the window's start
         c64.save_state()
-        c64.step(1)
+        c64.step(a.warmup + a.steps)
         c64.sync()
         c64.restore_state()
         c64.step(a.warmup)
