@@ -72,6 +72,12 @@ def test_null_arguments_rejected_without_gpu():
     out = C.c_void_p()
     assert L.sphe_create(9, C.byref(out)) == -1
     assert L.sphe_destroy(None) == -1
+    # per-scene entry points on a null context: no scene, nothing touched
+    dptr, n = C.c_void_p(), C.c_size_t()
+    assert L.sphe_positions_device(None, 0, C.byref(dptr), C.byref(n)) == -5
+    assert L.sphe_terrain_mesh_device(None, 0, C.byref(dptr), C.byref(n)) == -5
+    buf = (C.c_float * 3)()
+    assert L.sphe_read_buffer(None, 0, 0, buf, C.sizeof(buf)) == -5
 
 
 @pytest.mark.skipif(subprocess.run(["which", "g++"], capture_output=True).returncode != 0, reason="no g++")
