@@ -411,6 +411,12 @@ int sphe_debug_accel(sphe_ctx* ctx, int scene, double* acc);
  * known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters.
  * Returns the average ms per copy. */
 int sphe_debug_copy(sphe_ctx* ctx, size_t bytes, int iters, double* avg_ms);
+/* Starts the overflow pool's chunk claims at entry `entries` (a multiple of
+ * 16, at most half the pool's limit; 0 is the default) and allocates the
+ * pool that much larger: a test can place chunks past 2^32 entries and check
+ * that results do not change.  Rebuilds the device layout.  No counterpart in
+ * the reference. */
+int sphe_debug_pool_origin(sphe_ctx* ctx, int64_t entries);
 
 #ifdef __cplusplus
 }

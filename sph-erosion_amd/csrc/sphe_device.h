@@ -16,6 +16,10 @@
 
 #define SPHE_WAVE 64
 // cells per scan tile: 2^scan_log2 (256 threads x 8, 16 or 32 counts)
+// Overflow-pool chunk offsets are stored in units of this many entries, so
+// 32-bit offsets reach 2^36 entries.
+#define SPHE_POOL_UNIT 16
+
 #define SPHE_SCAN_LOG2_MIN 11
 #define SPHE_SCAN_LOG2_DEFAULT 12
 #define SPHE_SCAN_LOG2_MAX 13
@@ -110,14 +114,17 @@ struct DevState {
     uint32_t* nbc;
     int32_t maxn;
     // overflow pool: a particle whose count passes maxn claims one chunk of
-    // ocap more entries (pool[pbase[i] .. + ocap), neighbours maxn ..
+    // ocap more entries (pool[16 pbase[i] .. + ocap), neighbours maxn ..
     // maxn + ocap - 1) with an atomic on *ptop, which the scan of every step
-    // resets; pbase[i] is ~0u when it has none (the force pass then walks
-    // its stencil beyond maxn and sets status bit 4)
+    // resets to pool_origin; pbase[i] is the chunk's first entry in units of
+    // SPHE_POOL_UNIT entries (ocap and pool_origin are multiples of it), or
+    // ~0u when it has none (the force pass then walks its stencil beyond
+    // maxn and sets status bit 4).  pool_cap counts entries from pool[0].
     uint32_t* pool;
     unsigned long long* ptop;
     uint32_t* pbase;
     int64_t pool_cap;
+    int64_t pool_origin;
     int32_t ocap;
     // ovf[0]: largest neighbour count that overflowed a list; ovf[1]:
     // particles that found the pool full (both since the host last grew them)

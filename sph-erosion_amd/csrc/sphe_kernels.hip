@@ -622,8 +622,8 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* __restrict__ part
 // start[i] = exclusive prefix of counts, from partial[b]: the total of the
 // tiles before tile b (k_scan_tiles).  With zero_counts the counts are
 // reset to zero for the next arrivals (the particle sort instead has each
-// particle's scatter reset its own cell); the optional single word zero is
-// reset too.  occ (optional): the tile flags; an unflagged tile's counts are
+// particle's scatter reset its own cell); the optional single word *zero is
+// set to zero_to.  occ (optional): the tile flags; an unflagged tile's counts are
 // not read (all zero), and each block clears its tile's flag once read.
 // fill (optional, with occ): per tile, the value all of an empty tile's
 // starts hold since the tile was last written empty (~0: not known); an
@@ -632,9 +632,10 @@ template <int ITEMS>
 __global__ __launch_bounds__(256) void k_scan_down(uint32_t* __restrict__ cnt, uint32_t* __restrict__ start,
                                                    int64_t n, const uint32_t* __restrict__ partial,
                                                    int zero_counts, unsigned long long* __restrict__ zero,
+                                                   unsigned long long zero_to,
                                                    uint32_t* __restrict__ occ, uint32_t* __restrict__ fill)
 {
-    if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0ull;
+    if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = zero_to;
     __shared__ uint32_t lds[8];
     const int64_t base = (int64_t)blockIdx.x * (256 * ITEMS) + (int64_t)threadIdx.x * ITEMS;
     const bool empty = occ && occ[blockIdx.x] == 0u;  // block-uniform
@@ -992,12 +993,12 @@ __device__ __forceinline__ void density_body(const DevState& D, int b)
                     uint32_t b0 = ~0u;
                     if (l == 0) {
                         const unsigned long long t = atomicAdd(D.ptop, (unsigned long long)ocap);
-                        b0 = t + ocap <= (unsigned long long)D.pool_cap ? (uint32_t)t : ~0u;
+                        b0 = t + ocap <= (unsigned long long)D.pool_cap ? (uint32_t)(t / SPHE_POOL_UNIT) : ~0u;
                         if (b0 == ~0u) atomicAdd(&D.ovf[1], 1u);
                     }
                     pb = __shfl(b0, gfirst, SPHE_WAVE);
                 }
-                if (pb != ~0u && c0 - cap < ocap) dst = D.pool + pb + (c0 - cap);
+                if (pb != ~0u && c0 - cap < ocap) dst = D.pool + (size_t)pb * SPHE_POOL_UNIT + (c0 - cap);
             }
             if (dst) {
                 const uint32_t* src = (const uint32_t*)((char*)rowtab + (ring_rel | ((c0 * 4u) & RMASK))) + l * PER;
@@ -1528,9 +1529,9 @@ __device__ __forceinline__ void force_share(const DevState& D, const SceneDev& S
     }
     const uint32_t pb = D.pbase[slot];
     if (pb != ~0u && cnt - cap <= (uint32_t)D.ocap) {
-        const uint32_t* __restrict__ pool = D.pool;
+        const uint32_t* __restrict__ chunk = D.pool + (size_t)pb * SPHE_POOL_UNIT;
         force_list<T, SM, G>(D, K, l, cnt, pi, vi, a_i, acc,
-                               [&](uint32_t k) { return k < cap ? nbl[k] : pool[pb + (k - cap)]; });
+                               [&](uint32_t k) { return k < cap ? nbl[k] : chunk[k - cap]; });
         return;
     }
     // no room in list and pool: the stencil again (same order, same result)
@@ -2242,24 +2243,25 @@ void sphe_launch_hash(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent
 
 template <int ITEMS>
 static void launch_scan_t(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, bool zero_counts,
-                          unsigned long long* zero, uint32_t* occ, uint32_t* fill, hipStream_t st, hipEvent_t e0,
-                          hipEvent_t e1)
+                          unsigned long long* zero, unsigned long long zero_to, uint32_t* occ, uint32_t* fill,
+                          hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     const int64_t nb = (n + 256 * ITEMS - 1) / (256 * ITEMS);
     klaunch(k_scan_reduce<ITEMS>, (unsigned)nb, 256, st, e0, nullptr, (const uint32_t*)cnt, n, partial,
             (const uint32_t*)occ);
     klaunch(k_scan_tiles, 1, 1024, st, nullptr, nullptr, partial, nb);
     klaunch(k_scan_down<ITEMS>, (unsigned)nb, 256, st, nullptr, e1, cnt, start, n, (const uint32_t*)partial,
-            zero_counts ? 1 : 0, zero, occ, occ ? fill : nullptr);
+            zero_counts ? 1 : 0, zero, zero_to, occ, occ ? fill : nullptr);
 }
 
 void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, bool zero_counts,
-                      unsigned long long* zero, uint32_t* occ, uint32_t* fill, int scan_log2, hipStream_t st,
-                      hipEvent_t e0, hipEvent_t e1)
+                      unsigned long long* zero, unsigned long long zero_to, uint32_t* occ, uint32_t* fill,
+                      int scan_log2, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
-    if (scan_log2 == 11) launch_scan_t<8>(cnt, start, n, partial, zero_counts, zero, occ, fill, st, e0, e1);
-    else if (scan_log2 == 13) launch_scan_t<32>(cnt, start, n, partial, zero_counts, zero, occ, fill, st, e0, e1);
-    else launch_scan_t<16>(cnt, start, n, partial, zero_counts, zero, occ, fill, st, e0, e1);
+    if (scan_log2 == 11) launch_scan_t<8>(cnt, start, n, partial, zero_counts, zero, zero_to, occ, fill, st, e0, e1);
+    else if (scan_log2 == 13)
+        launch_scan_t<32>(cnt, start, n, partial, zero_counts, zero, zero_to, occ, fill, st, e0, e1);
+    else launch_scan_t<16>(cnt, start, n, partial, zero_counts, zero, zero_to, occ, fill, st, e0, e1);
 }
 
 int64_t sphe_scan_partials(int64_t n) { return (n + (1 << SPHE_SCAN_LOG2_MIN) - 1) >> SPHE_SCAN_LOG2_MIN; }
@@ -2355,7 +2357,7 @@ static void launch_scan_heights_t(const DevState& D, hipStream_t st, hipEvent_t 
     klaunch(k_scan_reduce_heights<ITEMS>, (unsigned)(nb + nh), 256, st, e0, nullptr, D, (int)nb);
     klaunch(k_scan_tiles, 1, 1024, st, nullptr, nullptr, D.partial, nb);
     klaunch(k_scan_down<ITEMS>, (unsigned)nb, 256, st, nullptr, e1, D.cnt, D.start, n, (const uint32_t*)D.partial,
-            0, D.ptop, D.tocc, D.tfill);
+            0, D.ptop, (unsigned long long)D.pool_origin, D.tocc, D.tfill);
 }
 
 void sphe_launch_scan_heights(const DevState& D, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
@@ -2375,7 +2377,7 @@ void sphe_launch_two_phase(const DevState& D, hipStream_t st, hipEvent_t e0, hip
 {
     if (D.TC == 0 || D.P == 0) return;
     klaunch(k_tcount, grid_of(D.P, 256), 256, st, e0, nullptr, D);
-    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, true, nullptr, nullptr, nullptr,
+    sphe_launch_scan(D.tcnt, D.tstart, D.TC + 1, D.tpartial, true, nullptr, 0ull, nullptr, nullptr,
                      SPHE_SCAN_LOG2_DEFAULT, st);
     k_tscatter<<<grid_of(D.P, 256), 256, 0, st>>>(D);
     klaunch(k_tapply, grid_of(D.TC, 256), 256, st, nullptr, e1, D);

@@ -8,11 +8,12 @@
 // e0 / e1 (optional): profiling events that take the start of the first and
 // the end of the last kernel a launcher issues.
 void sphe_launch_hash(const DevState& D, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// zero (optional): a word the scan sets to zero_to (the pool's claim counter);
 // occ (optional): one flag per tile of 2^scan_log2 counts, set where counts were taken;
 // fill (optional, with occ): per tile, the value an empty tile's starts already hold, or ~0
 void sphe_launch_scan(uint32_t* cnt, uint32_t* start, int64_t n, uint32_t* partial, bool zero_counts,
-                      unsigned long long* zero, uint32_t* occ, uint32_t* fill, int scan_log2, hipStream_t st,
-                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                      unsigned long long* zero, unsigned long long zero_to, uint32_t* occ, uint32_t* fill,
+                      int scan_log2, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // tiles of n counts at the smallest tile size (the size of the per-tile arrays)
 int64_t sphe_scan_partials(int64_t n);
 // The particle scan with the terrain heights computed in its first launch.

@@ -115,6 +115,7 @@ struct sphe_ctx {
     int ocap = 256;  // entries per overflow-pool chunk
     size_t mem_total = 0;  // the device's memory (bounds the neighbour lists)
     int64_t pool_chunks = 0;  // overflow chunks the last growth asked for (0: none yet)
+    int64_t pool_origin = 0;  // first pool entry the chunk claims start from (a test hook; 0)
     int lanes = 4;   // lanes per particle in the density and force passes
     int scan_log2 = SPHE_SCAN_LOG2_DEFAULT;  // cells per scan tile: 2^scan_log2
     bool persistent = false;  // density and force as persistent launches over work queues
@@ -197,11 +198,20 @@ hipError_t try_malloc(T** p, size_t bytes)
 
 size_t real_size(int precision) { return precision == SPHE_FP64 ? sizeof(double) : sizeof(float); }
 
-// Overflow pool entries for `chunks` chunks of ocap entries (pool offsets
-// are 32-bit).
-int64_t pool_entries(int64_t chunks, int ocap)
+// Largest overflow pool in entries, from pool[0]: a quarter of the device's
+// memory, and chunk offsets that fit 32 bits in units of SPHE_POOL_UNIT
+// entries (below the ~0u that means no chunk).
+int64_t pool_max(const sphe_ctx* c)
 {
-    return std::min<int64_t>(chunks * ocap, (int64_t)INT32_MAX);
+    const int64_t by_offset = ((int64_t)UINT32_MAX - 1) * SPHE_POOL_UNIT;
+    return std::min<int64_t>(by_offset, (int64_t)(c->mem_total / 4) / 4);
+}
+
+// Overflow pool entries for `chunks` chunks of ocap entries after the
+// context's pool origin.
+int64_t pool_entries(const sphe_ctx* c, int64_t chunks, int ocap)
+{
+    return std::min<int64_t>(c->pool_origin + chunks * ocap, pool_max(c));
 }
 
 // Overflow chunks of a new layout before any step has claimed one: one for
@@ -541,7 +551,9 @@ int build_layout(sphe_ctx* c)
     DALLOC(D.nbc, Pn * 4);
     D.ocap = c->ocap;
     // a rebuilt layout keeps the pool the steps before it sized
-    D.pool_cap = pool_entries(c->pool_chunks > 0 ? c->pool_chunks : pool_chunks_initial(P, D.ocap), D.ocap);
+    D.pool_origin = c->pool_origin;
+    D.pool_cap =
+        pool_entries(c, c->pool_chunks > 0 ? c->pool_chunks : pool_chunks_initial(P, D.ocap), D.ocap);
     DALLOC(D.pool, (size_t)D.pool_cap * 4);
     DALLOC(D.pbase, Pn * 4);
     DALLOC(D.ptop, 64);
@@ -836,8 +848,8 @@ void launch_sort(sphe_ctx* c)
         // with erosion the terrain heights of the last force pass are computed
         // in the scan's first launch (this step's force pass samples them)
         if (any_erosion(c, -1) && D.TC > 0) sphe_launch_scan_heights(D, c->stream, t.a, t.b);
-        else sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, false, D.ptop, D.tocc, D.tfill, D.scan_log2,
-                              c->stream, t.a, t.b);
+        else sphe_launch_scan(D.cnt, D.start, D.C + 1, D.partial, false, D.ptop,
+                              (unsigned long long)D.pool_origin, D.tocc, D.tfill, D.scan_log2, c->stream, t.a, t.b);
     }
     {
         Timed t(c, "scatter");
@@ -937,7 +949,7 @@ int resize_lists(sphe_ctx* c, int maxn, int ocap, int64_t chunks)
 {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const int64_t P = c->D.P;
-    const int64_t pool = pool_entries(chunks, ocap);
+    const int64_t pool = pool_entries(c, chunks, ocap);
     auto replace = [&](void* old, size_t old_bytes, size_t new_bytes, void** out) -> bool {
         auto it = std::find(c->allocs.begin(), c->allocs.end(), old);
         (void)hipFree(old);
@@ -977,7 +989,7 @@ int resize_lists(sphe_ctx* c, int maxn, int ocap, int64_t chunks)
     if (ok) c->D.ocap = ocap;
     c->maxn = c->D.maxn;
     c->ocap = c->D.ocap;
-    c->pool_chunks = c->D.pool_cap / c->D.ocap;
+    c->pool_chunks = (c->D.pool_cap - c->D.pool_origin) / c->D.ocap;
     ++c->epoch;  // the new lists hold nothing yet
     return SPHE_OK;
 }
@@ -1019,10 +1031,12 @@ int grow_lists(sphe_ctx* c)
     if (need > maxn) maxn = (int)std::max<int64_t>(maxn, std::min<int64_t>(up(need * 3 / 2, 128), list_max(c)));
     int ocap = c->D.ocap;
     if (need > maxn + ocap) ocap = (int)up((need - maxn) * 3 / 2, 128);
-    const int64_t claims = (int64_t)(claimed / (unsigned long long)c->D.ocap) + fails;
-    int64_t chunks = c->D.pool_cap / c->D.ocap;
+    const unsigned long long origin = (unsigned long long)c->D.pool_origin;
+    const int64_t claims =
+        (int64_t)((claimed > origin ? claimed - origin : 0ull) / (unsigned long long)c->D.ocap) + fails;
+    int64_t chunks = (c->D.pool_cap - c->D.pool_origin) / c->D.ocap;
     if (fails > 0 || ocap != c->D.ocap) chunks = std::max<int64_t>(64, up(claims * 3 / 2, 64));
-    chunks = std::min<int64_t>(chunks, INT32_MAX / ocap);
+    chunks = std::min<int64_t>(chunks, (pool_max(c) - c->D.pool_origin) / ocap);
     const int r = resize_lists(c, maxn, ocap, chunks);
     if (r) return r;
     HIPCHK(c, hipMemsetAsync(c->D.ovf, 0, 8, c->stream));
@@ -2145,7 +2159,21 @@ int sphe_get_overflow_capacity(const sphe_ctx* c, int* chunk_entries, int64_t* c
     if (!c) return SPHE_EARG;
     const int oc = c->layout_valid ? c->D.ocap : c->ocap;
     if (chunk_entries) *chunk_entries = oc;
-    if (chunks) *chunks = c->layout_valid ? c->D.pool_cap / oc : 0;
+    if (chunks) *chunks = c->layout_valid ? (c->D.pool_cap - c->D.pool_origin) / oc : 0;
+    return SPHE_OK;
+}
+
+int sphe_debug_pool_origin(sphe_ctx* c, int64_t entries)
+{
+    if (!c) return SPHE_EARG;
+    if (entries < 0 || entries % SPHE_POOL_UNIT || entries > pool_max(c) / 2)
+        return fail(c, SPHE_EARG, "pool origin must be a multiple of %d in [0, %lld]", SPHE_POOL_UNIT,
+                    (long long)(pool_max(c) / 2));
+    if (entries == c->pool_origin) return SPHE_OK;
+    // a new pool: the state goes through the host and the layout is rebuilt
+    int r = begin_host_edit(c);
+    if (r) return r;
+    c->pool_origin = entries;
     return SPHE_OK;
 }
 

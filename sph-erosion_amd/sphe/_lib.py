@@ -125,6 +125,7 @@ SIGNATURES = {
     "sphe_debug_accel": (I, [P, I, P]),
     "sphe_set_terrain_volume_fx": (I, [P, I, P, C.c_size_t]),
     "sphe_debug_copy": (I, [P, C.c_size_t, I, C.POINTER(C.c_double)]),
+    "sphe_debug_pool_origin": (I, [P, C.c_int64]),
 }
 
 _lib = None

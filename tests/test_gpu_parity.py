@@ -535,6 +535,31 @@ def test_list_overflow_paths_match_the_twin():
     assert ctx.status() == 0  # every neighbourhood fits now
 
 
+def test_overflow_chunks_past_32_bit_entry_offsets():
+    """Chunk claims that start at pool entry 2^32 + 2^20 (entry offsets past
+    32 bits; the chunks are addressed in units of 16 entries) give the bits
+    and the sizes of claims from entry 0, on every path of the overflow test
+    above: list, chunk, full pool and stencil walk, then grown lists."""
+    pos, p = _blob()
+    runs = []
+    for origin in (0, (1 << 32) + (1 << 20)):
+        ctx = sphe.Context()
+        sid = ctx.add_scene(p)
+        ctx.set_particles(sid, pos)
+        ctx.set_list_capacity(32)
+        assert sphe.lib().sphe_debug_pool_origin(ctx._p, origin) == 0
+        ctx.step(2)
+        first = (ctx.status(), ctx.state_f64(sid))
+        ctx.step(2)
+        runs.append((first, ctx.status(), ctx.state_f64(sid), ctx.get_list_capacity(), ctx.get_overflow_capacity()))
+        ctx.close()
+    (f0, st0, s0, cap0, oc0), (f1, st1, s1, cap1, oc1) = runs
+    assert f0[0] == f1[0] == 4  # some particles found the pool full and walked
+    for a, b in ((f0[1], f1[1]), (s0, s1)):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert (st0, cap0, oc0) == (st1, cap1, oc1)
+
+
 def test_list_capacity_never_changes_results():
     pos, p = _blob(2000, 7)
     out = []
