@@ -345,8 +345,8 @@ int sphe_get_yz_split(const sphe_ctx* ctx, int* split);
  * to 1.5x the largest count, as far as all lists together fit in a quarter
  * of the device's memory; longer neighbourhoods get pool chunks of 1.5x
  * their excess over the list, and the pool 1.5x the chunks the last step
- * claimed.  Setting the capacity sets the chunk size too.  Neither ever
- * changes results. */
+ * claimed, as far as it fits in another quarter.  Setting the capacity sets
+ * the chunk size too.  Neither ever changes results. */
 int sphe_set_list_capacity(sphe_ctx* ctx, int entries);
 int sphe_get_list_capacity(const sphe_ctx* ctx, int* entries);
 /* Entries per overflow chunk and the pool's chunk count (0 before the
