@@ -32,6 +32,11 @@ Prints one JSON line:
     restatement of it, the only use of the oracle here).
 Exits with status 3 when the step reported a non-finite state or a particle it
 could not place.
+
+`--mode horizon` instead runs the workload's whole trajectory from step 0 in
+chunks of 250 steps and prints one JSON line with the time per chunk and the
+conservation after each (exit status 3 if the units drift or bits 1 or 2 are
+set); `--mode copy` and `--mode summary` serve the rocprofv3 profiles.
 """
 import argparse
 import json
@@ -176,6 +181,48 @@ def front_end_loop(ctx, sids, scs, steps):
     n = sum(s.n for s in scs)
     return dict(steps=steps, readback_particles=n_big, readback_bytes_per_step=12 * n_big,
                 ms_per_step=1e3 * el / steps, particle_steps_per_s=n * steps / el)
+
+
+def horizon(workload, chunk):
+    """The workload's whole run from its initial state, as one context steps
+    it by default (FP32 compensated, untuned launch shapes), in chunks of
+    `chunk` steps: per chunk the time per step, the state of the neighbour
+    lists and pool, whether the fixed-point units are still exact, and the
+    FP64 relative drift of water plus sediment per scene."""
+    import sphe
+    scs, desc, run_steps = make_workload(workload)
+    ctx = sphe.Context(sphe.FP32)
+    sids = [ctx.add_synthetic(s) for s in scs]
+    t0m, s0m, u0m = ctx.mass_all()
+    tot0 = t0m + s0m
+    n = sum(s.n for s in scs)
+    chunks, done, total_s, exact, drift_max, bits = [], 0, 0.0, True, 0.0, 0
+    while done < run_steps:
+        k = min(chunk, run_steps - done)
+        ctx.sync()
+        t = time.perf_counter()
+        ctx.step(k)
+        ctx.sync()
+        el = time.perf_counter() - t
+        done += k
+        total_s += el
+        st = ctx.status()
+        t1m, s1m, u1m = ctx.mass_all()
+        drift = float((np.abs((t1m + s1m) - tot0) / np.where(tot0 > 0, tot0, 1.0)).max())
+        exact = exact and bool(np.array_equal(u0m, u1m))
+        drift_max = max(drift_max, drift)
+        bits |= st
+        chunks.append(dict(steps=done, ms_per_step=1e3 * el / k, status=st, units_exact=exact, drift=drift,
+                           list_capacity=ctx.get_list_capacity(), overflow=list(ctx.get_overflow_capacity())))
+        log(f"bench: horizon {workload} step {done}: {1e3 * el / k:.2f} ms/step, status {st}, "
+            f"units exact {exact}, drift {drift:.2e}")
+    ctx.close()
+    print(json.dumps(dict(mode="horizon", workload=f"{workload}: {desc}", particles=n, steps=run_steps,
+                          seconds=total_s, ms_per_step=1e3 * total_s / run_steps,
+                          particle_steps_per_s=n * run_steps / total_s, units_exact=exact,
+                          mass_drift_rel_max=drift_max, status_bits=bits, chunks=chunks)), flush=True)
+    if bits & 3 or not exact:
+        sys.exit(3)
 
 
 def accuracy_protocol(scs, start_state, tuning, warm=50):
@@ -416,9 +463,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-extras", action="store_true",
                     help="only the timed region (for profiler runs): no statistics, accuracy, FP64 or CPU legs")
-    ap.add_argument("--mode", default="step", choices=["step", "copy", "summary"],
+    ap.add_argument("--mode", default="step", choices=["step", "copy", "summary", "horizon"],
                     help="copy: time a plain 1 GiB device copy (HBM reference and counter calibration); "
-                         "summary: summarise the rocprofv3 runs under --prof-dir (no GPU)")
+                         "summary: summarise the rocprofv3 runs under --prof-dir (no GPU); "
+                         "horizon: the workload's whole run from step 0, timed in chunks, with its "
+                         "conservation checked at every chunk")
     ap.add_argument("--prof-dir", default="gpurun_out", help="where the rocprofv3 runs are (--mode summary)")
     ap.add_argument("--save-state", default=None,
                     help="write the state the timed window starts from (after settle and warm-up) to this .npz")
@@ -442,6 +491,9 @@ def main():
         nbytes = 1 << 30
         ms = ctx.debug_copy(nbytes, 20)
         print(json.dumps(dict(mode="copy", bytes=nbytes, avg_ms=ms, gbs=2 * nbytes / (ms * 1e-3) / 1e9)), flush=True)
+        return
+    if a.mode == "horizon":
+        horizon(a.workload, 250)
         return
     if a.no_extras:
         a.no_cpu = a.no_fp64 = a.no_tune = True
