@@ -468,7 +468,7 @@ def main():
                          "summary: summarise the rocprofv3 runs under --prof-dir (no GPU); "
                          "horizon: the workload's whole run from step 0, timed in chunks, with its "
                          "conservation checked at every chunk")
-    ap.add_argument("--prof-dir", default="gpurun_out", help="where the rocprofv3 runs are (--mode summary)")
+    ap.add_argument("--prof-dir", default=".", help="where the rocprofv3 runs are (--mode summary)")
     ap.add_argument("--save-state", default=None,
                     help="write the state the timed window starts from (after settle and warm-up) to this .npz")
     ap.add_argument("--load-state", default=None,
