@@ -49,7 +49,7 @@ extern "C" {
 #define SPHE_P_GRAV 4      /* FluidSystemSPH::GetGrav    fluid_system.h:281-284 (3 floats) */
 
 /* Per-particle buffer ids for sphe_read_buffer (the reference's FPOS..FPRESS
- * ids, fluid_system.h:22-26, plus the sediment this build adds). */
+ * ids, fluid_system.h:22-26, plus the sediment this library adds). */
 #define SPHE_BUF_POS 0      /* 3 floats */
 #define SPHE_BUF_VEL 1      /* 3 floats */
 #define SPHE_BUF_DENSITY 2  /* 1 float (of the last step) */
@@ -71,7 +71,7 @@ typedef struct sphe_params {
     float dt;            /* time step                        (deltaT, 0) */
     float box_center[3]; /* collision cube centre            (0,0,0) */
     float box_half;      /* collision cube half extent       (len, 0.2) */
-    /* erosion model (this build; the reference has none) */
+    /* erosion model (this library's; the reference has none) */
     int32_t erosion;     /* SPHE_EROSION_* */
     float kc;            /* sediment capacity per (m/s) of speed, times particle volume m/rho0 */
     float ke;            /* erode rate per step */

@@ -110,7 +110,7 @@ public:
         return p;
     }
 
-    // ---- additions of this build ------------------------------------------
+    // ---- additions beyond the reference's interface -------------------------
 
     // Positions, 3 floats per particle in id order (fluid_system.h:293-303,
     // getVertices): the buffer for one instanced draw in place of Draw.

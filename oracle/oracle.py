@@ -296,7 +296,7 @@ def heights(consts, vol_fx):
 
 
 def orc_step(scene, pos, vel, sed, ids, vol=None, height=None, real=8, comp=1, diag=True):
-    """One step of the build's algorithm on copies of the inputs.
+    """One step of the library's algorithm on copies of the inputs.
     Returns dict(pos, vel, sed, vol, height, active, and diag arrays)."""
     n = len(pos)
     pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(n, 3).copy()

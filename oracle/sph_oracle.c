@@ -14,19 +14,19 @@
  *                      plus closed forms (isolated particle, lattice density).
  *   orc_pred_threshold fluid_system.h:118-119 + glm length.  Pinned by an
  *                      exhaustive ulp walk against sqrtf and brute force.
- *   orc_row_width_f32  build-defined row half-width (Newton 1/sqrt, one
+ *   orc_row_width_f32  library-defined row half-width (Newton 1/sqrt, one
  *                      step).  Pinned by a dense sweep of [0, 1] (within
  *                      2e-3 of sqrt) and the clipped-row superset tests.
  *   orc_root_f32       length and normalize of fluid_system.h:118, 438-441
  *                      as the FP32 kernels evaluate them (Newton 1/sqrt).
  *                      Pinned by ulp bounds against sqrtf and 1/sqrtf over
  *                      the FP32 range, and d2 = 0 -> r = 0.
- *   orc_cell_keys /    build-defined grid search.  Pinned by set equality of
+ *   orc_cell_keys /    library-defined grid search.  Pinned by set equality of
  *   orc_step neighbours the grid neighbour sets with orc_neighbours_brute.
  *   orc_step (FP64)    the reference's SPH terms; pinned against ref_step on
  *                      the same states (relative tolerance, different sum
  *                      order and powf) and against closed forms.
- *   orc_step terrain   build-defined heightfield collision and erosion (the
+ *   orc_step terrain   library-defined heightfield collision and erosion (the
  *   and erosion        reference has none).  Pinned by closed-form single
  *                      particle cases and exact integer mass conservation.
  *   orc_terrain_mesh / grid.h:118-176 (UpdateGrid, genIndices) on a float
@@ -293,11 +293,11 @@ int ref_num_threads(void)
 }
 
 /* ====================================================================== */
-/* orc: the build's step                                                   */
+/* orc: the library's step                                                   */
 /* ====================================================================== */
 
 /* r = |d| and 1/r from d2 = |d|^2 as the FP32 kernels compute them
- * (build-defined): three Newton steps y <- y (1.5 - (d2/2) y^2) for 1/sqrt
+ * (library-defined): three Newton steps y <- y (1.5 - (d2/2) y^2) for 1/sqrt
  * from the integer estimate 0x5f375a86 - bits(d2)/2, each step a multiply,
  * a fused multiply-add and a multiply; r = d2 y.  Only IEEE operations, so
  * it replays the GPU bit for bit.  Pinned against correctly rounded sqrtf
@@ -319,7 +319,7 @@ void orc_root_f32(float d2, float* r, float* inv)
 }
 
 /* Half-width of a clipped stencil row before its margin, as the FP32 kernels
- * compute it (build-defined): x / sqrt(x) with one Newton step for 1/sqrt(x)
+ * compute it (library-defined): x / sqrt(x) with one Newton step for 1/sqrt(x)
  * from the integer estimate, within 2e-3 of sqrt(x) on [0, 1] (the 2^-6 row
  * margin covers it).  Pinned by a dense sweep of [0, 1] and by the
  * clipped-row superset tests. */
@@ -389,7 +389,7 @@ void orc_derive(const orc_scene* s, orc_consts* c)
         c->box_lo[a] = c->box_c[a] - c->box_e;
         c->box_hi[a] = c->box_c[a] + c->box_e;
     }
-    /* cells: h (1 + 2^-8) along x, 1/yz of that along y and z (build-defined) */
+    /* cells: h (1 + 2^-8) along x, 1/yz of that along y and z (library-defined) */
     c->yz = s->yz == 2 || s->yz == 3 ? s->yz : 1;
     c->cell = h * 1.00390625f; /* h (1 + 2^-8) */
     c->inv_cell = 1.0f / c->cell;

@@ -13,7 +13,7 @@
  *           reference's own header compiled into oracle/_ref and against the
  *           golden fixtures under tests/golden/.
  *
- *  orc_*    The build's step as specified for the GPU: canonical (cell, id)
+ *  orc_*    The library's step as specified for the GPU: canonical (cell, id)
  *           particle order, the reference's FP32 neighbour predicate, the
  *           reference's SPH terms with constants factored out of the sums,
  *           Kahan-compensated or plain sums, symplectic Euler with the
@@ -161,7 +161,7 @@ typedef struct {
     int32_t* nbcount;   /* neighbours including self */
 } orc_diag;
 
-/* One step of the build's algorithm for one scene, in place, input order
+/* One step of the library's algorithm for one scene, in place, input order
  * preserved.  real = 8 runs the FP64 oracle, real = 4 the FP32 twin.
  * comp (FP32 twin only): 0 plain sums, 1 Kahan-compensated density and force
  * sums, 2 compensated density sums and FP64-accumulated force sums.

@@ -15,7 +15,7 @@
 //             and the neighbour lists for the force pass
 //   force     pressure, viscosity, colour normal/Laplacian (:129-178),
 //             symplectic Euler + box response (:318-350, :355-407),
-//             heightfield collision and erosion exchange (build-defined),
+//             heightfield collision and erosion exchange (library-defined),
 //             and the cell count of the new position for the next step
 //   heights   terrain heights from the fixed-point volumes
 // The neighbour predicate is the reference's FP32 one, glm::length(r) <= h,
@@ -1364,7 +1364,7 @@ __device__ __forceinline__ void box_response(const SceneConsts<T>& K, T* p, T* v
 }
 
 // Bilinear height of the cell-centred terrain at (x, z) and the unit normal
-// of that patch (build-defined).
+// of that patch (library-defined).
 template <typename T>
 __device__ __forceinline__ T terrain_sample(const SceneDev& S, const SceneConsts<T>& K, const float* __restrict__ H,
                                             T x, T z, T* nrm)
@@ -1418,7 +1418,7 @@ __device__ __forceinline__ void terrain_collide(const SceneConsts<T>& K, T hs, c
     p[1] = hs;
 }
 
-// Erosion / deposition exchange of one particle (build-defined).  Returns
+// Erosion / deposition exchange of one particle (library-defined).  Returns
 // the fixed-point amount taken from terrain cell *tc (negative: deposited),
 // or 0 with *tc = -1 outside the contact band.
 template <typename T>

@@ -1,5 +1,5 @@
-"""The oracle's grid step (the build's algorithm) against brute force, the
-reference restatement and closed forms of the build-defined terrain and
+"""The oracle's grid step (the library's algorithm) against brute force, the
+reference restatement and closed forms of the library-defined terrain and
 erosion model."""
 import os
 
