@@ -477,6 +477,8 @@ def main():
     ap.add_argument("--grid", default=None, help="y/z split, x slabs, log2 scan tile, e.g. 2,4,12 (with --no-tune)")
     ap.add_argument("--block", type=int, default=None, help="density block size (with --no-tune)")
     ap.add_argument("--force-block", type=int, default=None, help="force block size (with --no-tune)")
+    ap.add_argument("--sequence-steps", type=int, default=None,
+                    help="steps per captured graph launch (the library's default if not given)")
     a = ap.parse_args()
     if a.mode == "summary":
         summarize_profiles(a.prof_dir, a.workload, a.steps)
@@ -505,6 +507,8 @@ def main():
     N = sum(s.n for s in scs)
 
     ctx = sphe.Context(sphe.FP32)
+    if a.sequence_steps:
+        ctx.set_sequence_steps(a.sequence_steps)
     sids = [ctx.add_synthetic(s) for s in scs]
     if a.no_tune:
         if a.grid:
@@ -607,7 +611,8 @@ def main():
                                   f"({settle} settle + {a.warmup} warm-up steps before it, untimed)",
                            precision="fp32 (Kahan-compensated density and force sums)", block=blk,
                            force_block=fblk, density_xcd_run=rm, x_slabs=ctx.get_subcells(),
-                           yz_split=ctx.get_yz_split(), scan_tile_log2=ctx.get_scan_tile()),
+                           yz_split=ctx.get_yz_split(), scan_tile_log2=ctx.get_scan_tile(),
+                           sequence_steps=ctx.get_sequence_steps()),
                roofline=roofline(dom, kernels[dom], counters))
     out["roofline_force"] = roofline("force", kernels["force"], counters)
     out["value_kernel_by_kernel"] = N * a.steps / prof_elapsed

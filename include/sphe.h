@@ -362,6 +362,13 @@ int sphe_autotune(sphe_ctx* ctx, int steps_per_trial);
  * the reference. */
 int sphe_set_persistent(sphe_ctx* ctx, int on);
 int sphe_get_persistent(const sphe_ctx* ctx, int* on);
+/* Steps per captured graph launch of a long sphe_step call (1..4096,
+ * default 32); the remaining steps of a call run as one-step graphs.  The
+ * host reads the list-overflow words between launches, so longer sequences
+ * grow the lists later.  Results do not depend on it.  No counterpart in
+ * the reference (one Run per frame, fluid_system.h:104-183). */
+int sphe_set_sequence_steps(sphe_ctx* ctx, int steps);
+int sphe_get_sequence_steps(const sphe_ctx* ctx, int* steps);
 /* Cells per scan tile of the sort's prefix sum: 2^log2_cells, 11..13
  * (default 12).  Results do not depend on it.  No counterpart in the
  * reference (which has no grid). */

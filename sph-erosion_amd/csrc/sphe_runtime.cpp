@@ -116,6 +116,7 @@ struct sphe_ctx {
     size_t mem_total = 0;  // the device's memory (bounds the neighbour lists)
     int64_t pool_chunks = 0;  // overflow chunks the last growth asked for (0: none yet)
     int64_t pool_origin = 0;  // first pool entry the chunk claims start from (a test hook; 0)
+    int seq_steps = 32;       // steps per captured graph launch of a long sphe_step call
     int lanes = 4;   // lanes per particle in the density and force passes
     int scan_log2 = SPHE_SCAN_LOG2_DEFAULT;  // cells per scan tile: 2^scan_log2
     bool persistent = false;  // density and force as persistent launches over work queues
@@ -1045,8 +1046,6 @@ int grow_lists(sphe_ctx* c)
     return SPHE_OK;
 }
 
-// Steps per captured graph launch of a long call.
-const int kSeqSteps = 32;
 // Launches queued ahead of the one whose published overflow word the host
 // reads (the size of sphe_ctx::seq_ev): the device never runs dry, and lists
 // too short for the state grow at most this many launches late.
@@ -1064,7 +1063,7 @@ int do_steps(sphe_ctx* c, int nsteps)
     }
     HIPCHK(c, hipGetLastError());
     const bool graph = c->use_graph && !c->profiling;
-    // Graph mode: sequences of kSeqSteps steps in one graph launch, the rest
+    // Graph mode: sequences of c->seq_steps steps in one graph launch, the rest
     // one step per launch; direct launches (profiling, or graphs off) one
     // step at a time.  Before each launch the host looks at the overflow word
     // the last finished sequence published.
@@ -1073,7 +1072,7 @@ int do_steps(sphe_ctx* c, int nsteps)
         if (c->seq_count >= (uint64_t)kInFlight) HIPCHK(c, hipEventSynchronize(ev));
         r = grow_lists(c);
         if (r) return r;
-        const int k = graph && nsteps - done >= kSeqSteps ? kSeqSteps : 1;
+        const int k = graph && nsteps - done >= c->seq_steps ? c->seq_steps : 1;
         if (graph) {
             hipGraphExec_t x = nullptr;
             r = ensure_graph(c, k, &x);
@@ -2160,6 +2159,21 @@ int sphe_get_overflow_capacity(const sphe_ctx* c, int* chunk_entries, int64_t* c
     const int oc = c->layout_valid ? c->D.ocap : c->ocap;
     if (chunk_entries) *chunk_entries = oc;
     if (chunks) *chunks = c->layout_valid ? (c->D.pool_cap - c->D.pool_origin) / oc : 0;
+    return SPHE_OK;
+}
+
+int sphe_set_sequence_steps(sphe_ctx* c, int steps)
+{
+    if (!c) return SPHE_EARG;
+    if (steps < 1 || steps > 4096) return fail(c, SPHE_EARG, "steps per sequence must be in [1, 4096]");
+    c->seq_steps = steps;  // graphs are kept per length: nothing else changes
+    return SPHE_OK;
+}
+
+int sphe_get_sequence_steps(const sphe_ctx* c, int* steps)
+{
+    if (!c || !steps) return SPHE_EARG;
+    *steps = c->seq_steps;
     return SPHE_OK;
 }
 

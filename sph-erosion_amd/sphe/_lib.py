@@ -106,6 +106,8 @@ SIGNATURES = {
     "sphe_set_scan_tile": (I, [P, I]),
     "sphe_set_persistent": (I, [P, I]),
     "sphe_get_persistent": (I, [P, C.POINTER(I)]),
+    "sphe_set_sequence_steps": (I, [P, I]),
+    "sphe_get_sequence_steps": (I, [P, C.POINTER(I)]),
     "sphe_get_scan_tile": (I, [P, C.POINTER(I)]),
     "sphe_set_lanes": (I, [P, I]),
     "sphe_get_lanes": (I, [P, C.POINTER(I)]),

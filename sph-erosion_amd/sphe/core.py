@@ -378,6 +378,14 @@ class Context:
         self._chk(lib().sphe_get_persistent(self._p, C.byref(v)), "get_persistent")
         return bool(v.value)
 
+    def set_sequence_steps(self, steps):
+        self._chk(lib().sphe_set_sequence_steps(self._p, steps), "set_sequence_steps")
+
+    def get_sequence_steps(self):
+        v = C.c_int()
+        self._chk(lib().sphe_get_sequence_steps(self._p, C.byref(v)), "get_sequence_steps")
+        return v.value
+
     def set_scan_tile(self, log2_cells):
         self._chk(lib().sphe_set_scan_tile(self._p, log2_cells), "set_scan_tile")
 

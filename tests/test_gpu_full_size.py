@@ -193,19 +193,28 @@ def test_cfg3_replay_bit_identical(cfg3_run):
 
 @pytest.mark.timeout(300)
 def test_cfg3_step_sequences_equal_direct_launches():
-    """75 steps as two captured 32-step sequences plus single-step graphs
-    equal 75 steps launched kernel by kernel, for all 256 scenes."""
+    """The whole 300-step run as one captured 300-step sequence, as nine
+    32-step sequences plus single-step graphs, and launched kernel by kernel
+    ends in the same bits for all 256 scenes."""
     scs = scenes.cfg3()
-    a = sphe.Context(sphe.FP32)
-    b = sphe.Context(sphe.FP32)
-    b.use_graph(False)
-    sa = [a.add_synthetic(s) for s in scs]
-    sb = [b.add_synthetic(s) for s in scs]
-    a.step(75)
-    b.step(75)
-    assert a.status() & 3 == 0 and b.status() & 3 == 0
-    for x, y in zip(sa, sb):
-        assert same_state(snapshot(a, x), snapshot(b, y))
+    ctxs = []
+    for mode in ("seq300", "seq32", "direct"):
+        c = sphe.Context(sphe.FP32)
+        if mode == "seq300":
+            c.set_sequence_steps(300)
+            assert c.get_sequence_steps() == 300
+        if mode == "direct":
+            c.use_graph(False)
+        sids = [c.add_synthetic(s) for s in scs]
+        c.step(300)
+        assert c.status() & 3 == 0
+        ctxs.append((c, sids))
+    (a, sa), others = ctxs[0], ctxs[1:]
+    for b, sb in others:
+        for x, y in zip(sa, sb):
+            assert same_state(snapshot(a, x), snapshot(b, y))
+    for c, _ in ctxs:
+        c.close()
 
 
 # ------------------------------------------------------------------ cfg4
