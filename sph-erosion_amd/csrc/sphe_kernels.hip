@@ -688,8 +688,11 @@ __global__ __launch_bounds__(256) void k_scatter(DevState D)
         atomicOr(D.status, 2u);
         return;
     }
-    const uint32_t d = D.start[c] + D.arank[p];
-    D.cnt[c] = 0u;  // the cell's count is used up: reset for the next step's arrivals
+    const uint32_t r = D.arank[p];
+    const uint32_t d = D.start[c] + r;
+    // the cell's count is used up: its first arrival (one per counted cell)
+    // resets it for the next step's arrivals
+    if (r == 0u) D.cnt[c] = 0u;
     if ((int64_t)d >= D.P) {  // counts inconsistent with the slots: report, never write out of range
         atomicOr(D.status, 2u);
         return;
