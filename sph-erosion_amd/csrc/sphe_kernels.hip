@@ -741,9 +741,20 @@ __global__ __launch_bounds__(256) void k_reorder(DevState D)
         whi = 0u;
     }
     __syncthreads();
-    if (ok && s0 < s1) {
-        atomicMin(&wlo, s0);
-        atomicMax(&whi, s1);
+    {
+        // the block's window of cell slots: reduced over each wave, then one
+        // LDS atomic pair per wave
+        uint32_t lo = ok && s0 < s1 ? s0 : ~0u, hi = ok && s0 < s1 ? s1 : 0u;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const uint32_t a = __shfl_xor(lo, o, 64), b = __shfl_xor(hi, o, 64);
+            lo = a < lo ? a : lo;
+            hi = b > hi ? b : hi;
+        }
+        if ((threadIdx.x & 63) == 0 && lo < hi) {
+            atomicMin(&wlo, lo);
+            atomicMax(&whi, hi);
+        }
     }
     __syncthreads();
     const uint32_t w0 = wlo, w1 = whi;
