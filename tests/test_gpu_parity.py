@@ -79,10 +79,12 @@ def test_density_vs_fp64_and_twin(precision, comp):
 @pytest.mark.parametrize("erosion,precision,comp", [(sphe.EROSION_OFF, sphe.FP32, 1),
                                                      (sphe.EROSION_ATOMIC, sphe.FP32, 1),
                                                      (sphe.EROSION_TWO_PHASE, sphe.FP32, 1),
-                                                     (sphe.EROSION_ATOMIC, sphe.FP32_NAIVE, 0)])
+                                                     (sphe.EROSION_ATOMIC, sphe.FP32_NAIVE, 0),
+                                                     (sphe.EROSION_ATOMIC, sphe.FP32_F64ACC, 2)])
 def test_step_fp32_bit_exact_twin(erosion, precision, comp):
     """Three steps equal the twin's bit for bit (positions, velocities,
-    sediment, terrain volumes and heights), compensated and plain FP32."""
+    sediment, terrain volumes and heights): compensated FP32, plain FP32, and
+    FP32 terms with FP64 force sums."""
     sc = small_dam(erosion=erosion)
     ctx, sid = gpu_context(sc, precision)
     pos, vel, sed, vol, height, fx = oracle_inputs(ctx, sid, sc)
