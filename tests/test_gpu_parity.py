@@ -336,18 +336,22 @@ def test_slab_split_bit_exact_twin(slabs):
     assert np.array_equal(ctx.sediment_fx(sid), st["sed"])
 
 
-@pytest.mark.parametrize("yz", [1, 3])
-def test_yz_split_bit_exact_twin(yz):
+@pytest.mark.parametrize("yz,lanes", [(1, 4), (3, 4), (3, 8), (3, 1), (1, 8)])
+def test_yz_split_bit_exact_twin(yz, lanes):
     """The y/z cell split sets the grid, the canonical order and the clipped
     rows; the twin with the same split reproduces the GPU steps bit for bit
-    (the default split is covered by every other twin test)."""
+    (the default split is covered by every other twin test).  Lane counts
+    other than 4 walk the rows in rounds of their own width: 8 lanes step
+    past a whole stencil row of split 3 (7 rows) at once."""
     sc = small_dam()
     ctx, sid = gpu_context(sc)
     ctx.set_yz_split(yz)
+    ctx.set_lanes(lanes)
     assert ctx.get_yz_split() == yz
     pos, vel, sed, vol, height, fx = oracle_inputs(ctx, sid, sc)
     osc = orc_scene_from(sc.params, sc.terrain, fx)
     osc.yz = yz
+    osc.lanes = lanes
     counts, off, idx = ctx.debug_neighbours(sid)
     bc, boff, bidx = O.neighbours_brute(sc.pos.astype(np.float32), O.derive(osc).thr)
     assert np.array_equal(counts, bc) and np.array_equal(idx, bidx)
