@@ -316,17 +316,20 @@ def test_lane_split_bit_exact_twin(lanes):
     assert np.array_equal(ctx.sediment_fx(sid), st["sed"])
 
 
-@pytest.mark.parametrize("slabs", [1, 2])
-def test_slab_split_bit_exact_twin(slabs):
+@pytest.mark.parametrize("slabs,yz", [(1, 2), (2, 2), (1, 3), (2, 1), (1, 1)])
+def test_slab_split_bit_exact_twin(slabs, yz):
     """The x-slab count sets the canonical order and the clipped rows; the
-    twin with the same count reproduces the GPU steps bit for bit."""
+    twin with the same count and y/z split reproduces the GPU steps bit for
+    bit."""
     sc = small_dam()
     ctx, sid = gpu_context(sc)
     ctx.set_subcells(slabs)
+    ctx.set_yz_split(yz)
     assert ctx.get_subcells() == slabs
     pos, vel, sed, vol, height, fx = oracle_inputs(ctx, sid, sc)
     osc = orc_scene_from(sc.params, sc.terrain, fx)
     osc.slabs = slabs
+    osc.yz = yz
     ctx.step(2)
     gp, gv = ctx.state_f64(sid)
     st = dict(pos=pos, vel=vel, sed=sed, vol=vol, height=height)
