@@ -122,11 +122,16 @@ def test_step_fp32_vs_fp64_oracle():
     assert dv <= 1e-4 * vscale, (dv, vscale)
 
 
-def test_fp64_path_vs_oracle():
+@pytest.mark.parametrize("yz", [2, 1, 3])
+def test_fp64_path_vs_oracle(yz):
+    """The plain FP64 path against the FP64 oracle on the same grid (every
+    y/z split: the FP64 kernels have their own row and stencil code)."""
     sc = small_dam()
     ctx, sid = gpu_context(sc, sphe.FP64)
+    ctx.set_yz_split(yz)
     pos, vel, sed, vol, height, fx = oracle_inputs(ctx, sid, sc)
     osc = orc_scene_from(sc.params, sc.terrain, fx)
+    osc.yz = yz
     ctx.step(2)
     gp, gv = ctx.state_f64(sid)
     st = dict(pos=pos, vel=vel, sed=sed, vol=vol, height=height)
