@@ -574,6 +574,32 @@ def test_overflow_chunks_past_32_bit_entry_offsets():
     assert (st0, cap0, oc0) == (st1, cap1, oc1)
 
 
+def test_list_overflow_paths_fp64():
+    """The FP64 kernels on the overflow test's blob with 32-entry lists (list,
+    chunk, full pool and stencil walk) against the FP64 oracle, and the same
+    bits as with 1,024-entry lists (a different mix of the three paths)."""
+    pos, p = _blob()
+    n = len(pos)
+    runs = []
+    for cap in (32, 1024):
+        ctx = sphe.Context(sphe.FP64)
+        sid = ctx.add_scene(p)
+        ctx.set_particles(sid, pos)
+        ctx.set_list_capacity(cap)
+        ctx.step(2)
+        runs.append((ctx.status(), ctx.state_f64(sid)))
+        ctx.close()
+    assert runs[0][0] == 4  # particles past list and pool walked their stencil
+    assert np.array_equal(runs[0][1][0], runs[1][1][0]) and np.array_equal(runs[0][1][1], runs[1][1][1])
+    osc = orc_scene_from(p)
+    st = dict(pos=pos.astype(np.float64), vel=np.zeros((n, 3)), sed=np.zeros(n, np.int64))
+    for _ in range(2):
+        st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], np.arange(n, dtype=np.int32), real=8)
+    gp, gv = runs[0][1]
+    assert np.abs(gp - st["pos"]).max() <= 1e-12
+    assert np.abs(gv - st["vel"]).max() <= 1e-9 * max(1.0, np.abs(st["vel"]).max())
+
+
 def test_list_capacity_never_changes_results():
     pos, p = _blob(2000, 7)
     out = []
