@@ -175,11 +175,14 @@ def test_mass_conservation_exact():
     assert abs((t1 + s1) - (t0 + s0)) <= 1e-9 * (t0 + s0)
 
 
-def test_atomic_equals_two_phase():
+@pytest.mark.parametrize("precision", [sphe.FP32, sphe.FP64])
+def test_atomic_equals_two_phase(precision):
+    """Atomic and two-phase erosion give the same bits over 20 steps (integer
+    exchanges: their order does not matter), in FP32 and in FP64."""
     a = small_dam(erosion=sphe.EROSION_ATOMIC)
     b = small_dam(erosion=sphe.EROSION_TWO_PHASE)
-    ca, sa = gpu_context(a)
-    cb, sb = gpu_context(b)
+    ca, sa = gpu_context(a, precision)
+    cb, sb = gpu_context(b, precision)
     ca.step(20)
     cb.step(20)
     assert np.array_equal(ca.terrain_volume_fx(sa), cb.terrain_volume_fx(sb))
