@@ -653,6 +653,36 @@ def test_debug_accel_is_the_steps_acceleration():
     assert np.abs(a64 - ref).max() <= 1e-9 * scale
 
 
+@pytest.mark.parametrize("yz,lanes,precision,comp", [(3, 8, sphe.FP32, 1), (1, 1, sphe.FP32, 1),
+                                                      (2, 4, sphe.FP32_F64ACC, 2), (3, 2, sphe.FP32_NAIVE, 0)])
+def test_inspection_kernels_bit_exact_on_every_grid(yz, lanes, precision, comp):
+    """The inspection kernels repeat the step's sums in its order on every
+    split, lane count and FP32 mode: sphe_debug_accel (the next step's
+    acceleration) and GetParticle (the last step's acceleration and density)
+    equal the twin's bit for bit."""
+    sc = small_dam()
+    ctx, sid = gpu_context(sc, precision)
+    ctx.set_yz_split(yz)
+    ctx.set_lanes(lanes)
+    pos, vel, sed, vol, height, fx = oracle_inputs(ctx, sid, sc)
+    osc = orc_scene_from(sc.params, sc.terrain, fx)
+    osc.yz = yz
+    osc.lanes = lanes
+    twin = O.orc_step(osc, pos, vel, sed, sc.ids, vol, height, real=4, comp=comp)
+    assert np.array_equal(ctx.debug_accel(sid), twin["acc"])
+    ctx.step(2)
+    st = dict(pos=pos, vel=vel, sed=sed, vol=vol, height=height)
+    for _ in range(2):
+        st = O.orc_step(osc, st["pos"], st["vel"], st["sed"], sc.ids, st["vol"], st["height"], real=4, comp=comp)
+    for pid in (0, 1234, 4095):
+        rec = ctx.particle(sid, pid)
+        assert np.array_equal(np.array(rec.Acceleration[:], dtype=np.float32), st["acc"][pid].astype(np.float32))
+        assert np.float32(rec.Density) == np.float32(st["rho"][pid])
+    nxt = O.orc_step(osc, st["pos"], st["vel"], st["sed"], sc.ids, st["vol"], st["height"], real=4, comp=comp)
+    assert np.array_equal(ctx.debug_accel(sid), nxt["acc"])
+    ctx.close()
+
+
 def test_step_integrates_the_reported_acceleration():
     """v' = v + a dt in FP32 with a from sphe_debug_accel, bit for bit, for
     every particle the step leaves strictly inside the box (no terrain)."""
